@@ -1,0 +1,249 @@
+"""Python mirror of the CULS AIG manager surface over libgls.so (ctypes).
+
+``Aig`` follows the reference's ``AIGMan`` (src/aig_manager.h:8-100): read /
+write an AIGER file, ``strash``, ``rewrite(use_zeros)``, stats as printed by
+``ps``.  All compute goes through the C ABI in include/gls.h; if the library is
+missing this module raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgls.so")
+
+GLS_LIT_AIGER = 0
+GLS_LIT_CULS = 1
+
+
+class GlsError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("nPIs", ctypes.c_int32), ("nPOs", ctypes.c_int32), ("nAnds", ctypes.c_int32),
+                ("nObjs", ctypes.c_int32), ("levels", ctypes.c_int32), ("on_device", ctypes.c_int32)]
+
+
+class OpInfo(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("reorder_ms", ctypes.c_double), ("cuts_ms", ctypes.c_double),
+                ("table_ms", ctypes.c_double), ("eval_ms", ctypes.c_double), ("select_ms", ctypes.c_double),
+                ("build_ms", ctypes.c_double), ("strash_ms", ctypes.c_double),
+                ("candidates", ctypes.c_int64), ("selected", ctypes.c_int64), ("new_nodes", ctypes.c_int64),
+                ("rounds", ctypes.c_int64), ("levels_enumerated", ctypes.c_int64), ("est_gain", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_SIGS = {
+    "gls_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "gls_ctx_destroy": (None, [_P]),
+    "gls_last_error": (ctypes.c_char_p, [_P]),
+    "gls_ctx_has_device": (ctypes.c_int, [_P]),
+    "gls_aig_from_arrays": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
+                                           ctypes.c_int32, ctypes.POINTER(_P)]),
+    "gls_aig_to_arrays": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32]),
+    "gls_aig_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "gls_aig_copy": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "gls_aig_free": (None, [_P]),
+    "gls_aig_read_aiger": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "gls_aig_read_aiger_mem": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "gls_aig_write_aiger": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "gls_strash": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "gls_rewrite": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "gls_aig_last_op": (ctypes.c_int, [_P, ctypes.POINTER(OpInfo)]),
+    "gls_reorder": (ctypes.c_int, [_P, _P]),
+    "gls_enumerate_cuts": (ctypes.c_int, [_P, _P]),
+    "gls_evaluate": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
+    "gls_npn_tables": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gls_gen_random": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "gls_gen_geometric": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "gls_gen_multiplier": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.POINTER(_P)]),
+}
+
+
+def lib():
+    """Loads libgls.so (raises GlsError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GlsError("libgls.so is not built: run `python -m culs_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Context:
+    """One HIP device + stream (device=-1: host-only: I/O and generators)."""
+
+    def __init__(self, device=0):
+        self._h = _P()
+        rc = lib().gls_ctx_create(device, ctypes.byref(self._h))
+        if rc != 0:
+            raise GlsError("gls_ctx_create(%d) failed with %d" % (device, rc))
+        self.device = device
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().gls_last_error(self._h)
+            raise GlsError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    @property
+    def has_device(self):
+        return bool(lib().gls_ctx_has_device(self._h))
+
+    def close(self):
+        if self._h:
+            lib().gls_ctx_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # constructors
+    def from_arrays(self, nPIs, f0, f1, pos, convention=GLS_LIT_AIGER):
+        f0 = np.ascontiguousarray(f0, dtype=np.uint32)
+        f1 = np.ascontiguousarray(f1, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos, dtype=np.uint32)
+        n = len(f0)
+        h = _P()
+        self.check(lib().gls_aig_from_arrays(self._h, nPIs, len(pos), n - nPIs - 1, _ptr(f0), _ptr(f1),
+                                             _ptr(pos) if len(pos) else None, convention, ctypes.byref(h)),
+                   "gls_aig_from_arrays")
+        return Aig(self, h)
+
+    def read_aiger(self, path):
+        h = _P()
+        self.check(lib().gls_aig_read_aiger(self._h, path.encode(), ctypes.byref(h)), "gls_aig_read_aiger")
+        return Aig(self, h)
+
+    def read_aiger_bytes(self, data):
+        h = _P()
+        self.check(lib().gls_aig_read_aiger_mem(self._h, data, len(data), ctypes.byref(h)), "read_aiger_mem")
+        return Aig(self, h)
+
+    def gen_random(self, nPIs, nAnds, nPOs, window, seed):
+        h = _P()
+        self.check(lib().gls_gen_random(self._h, nPIs, nAnds, nPOs, window, seed, ctypes.byref(h)), "gen_random")
+        return Aig(self, h)
+
+    def gen_geometric(self, nPIs, nAnds, nPOs, window, p, seed):
+        h = _P()
+        self.check(lib().gls_gen_geometric(self._h, nPIs, nAnds, nPOs, window, p, seed, ctypes.byref(h)),
+                   "gen_geometric")
+        return Aig(self, h)
+
+    def gen_multiplier(self, nbits, redundant, seed):
+        h = _P()
+        self.check(lib().gls_gen_multiplier(self._h, nbits, redundant, seed, ctypes.byref(h)), "gen_multiplier")
+        return Aig(self, h)
+
+
+class Aig:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self._h = handle
+
+    def free(self):
+        if self._h:
+            lib().gls_aig_free(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        self.ctx.check(lib().gls_aig_stats(self._h, ctypes.byref(s)), "gls_aig_stats")
+        return s
+
+    @property
+    def n_ands(self):
+        return self.stats().nAnds
+
+    def arrays(self, convention=GLS_LIT_AIGER):
+        """(nPIs, fanin0, fanin1, pos) as uint32 numpy arrays indexed by id."""
+        s = self.stats()
+        f0 = np.zeros(s.nObjs, np.uint32)
+        f1 = np.zeros(s.nObjs, np.uint32)
+        pos = np.zeros(max(s.nPOs, 1), np.uint32)
+        self.ctx.check(lib().gls_aig_to_arrays(self._h, _ptr(f0), _ptr(f1), _ptr(pos), convention),
+                       "gls_aig_to_arrays")
+        return s.nPIs, f0, f1, pos[:s.nPOs]
+
+    def copy(self):
+        h = _P()
+        self.ctx.check(lib().gls_aig_copy(self._h, ctypes.byref(h)), "gls_aig_copy")
+        return Aig(self.ctx, h)
+
+    def write_aiger(self, path):
+        self.ctx.check(lib().gls_aig_write_aiger(self._h, path.encode()), "gls_aig_write_aiger")
+
+    def strash(self):
+        self.ctx.check(lib().gls_strash(self._h, 0), "gls_strash")
+        return self
+
+    def rewrite(self, use_zeros=False):
+        self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, 0), "gls_rewrite")
+        return self
+
+    def last_op(self):
+        info = OpInfo()
+        self.ctx.check(lib().gls_aig_last_op(self._h, ctypes.byref(info)), "gls_aig_last_op")
+        return info.as_dict()
+
+    def reorder(self):
+        s = self.stats()
+        lev = np.zeros(max(s.nObjs, 1), np.uint32)
+        self.ctx.check(lib().gls_reorder(self._h, _ptr(lev)), "gls_reorder")
+        return lev[:self.stats().nObjs]
+
+    def enumerate_cuts(self):
+        s = self.stats()
+        cuts = np.zeros(s.nObjs * 40, np.uint32)
+        self.ctx.check(lib().gls_enumerate_cuts(self._h, _ptr(cuts)), "gls_enumerate_cuts")
+        return cuts
+
+    def evaluate(self, use_zeros=False):
+        s = self.stats()
+        best = np.zeros(s.nObjs * 4, np.int32)
+        self.ctx.check(lib().gls_evaluate(self._h, 1 if use_zeros else 0, _ptr(best)), "gls_evaluate")
+        return best
+
+    def ps(self):
+        """The reference's `ps` line (src/aig_manager.cu:604-612)."""
+        s = self.stats()
+        return "AIG stats: i/o = %d/%d and = %d level = %d" % (s.nPIs, s.nPOs, s.nAnds, s.levels)
+
+
+def npn_tables():
+    ph = np.zeros(65536, np.uint8)
+    pe = np.zeros(65536, np.uint8)
+    cl = np.zeros(65536, np.uint8)
+    p4 = np.zeros(96, np.uint8)
+    rc = lib().gls_npn_tables(_ptr(ph), _ptr(pe), _ptr(cl), _ptr(p4))
+    if rc != 0:
+        raise GlsError("gls_npn_tables failed")
+    return ph, pe, cl, p4.reshape(24, 4)

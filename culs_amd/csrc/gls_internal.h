@@ -1,0 +1,110 @@
+// Internal declarations shared by the host C++ and the HIP translation units of
+// libgls.so.  Not part of the C ABI.
+#pragma once
+#include "../../include/gls.h"
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#define GLS_LID(l) ((l) >> 1)
+#define GLS_LC(l) ((l) & 1u)
+#define GLS_INVALID 0xFFFFFFFFu
+
+// Records the first failure in the context and returns its code.
+int gls_fail(gls_ctx *ctx, int code, const char *fmt, ...);
+
+#define GLS_HIP(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return gls_fail((ctx), GLS_ERR_HIP, "%s failed: %s (%s:%d)", #call,            \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define GLS_TRY(expr)                                                                       \
+    do {                                                                                    \
+        int r_ = (expr);                                                                    \
+        if (r_ != GLS_OK) return r_;                                                        \
+    } while (0)
+
+// Device memory with a size-bucketed free list: passes of rw/strash allocate
+// and release many temporaries of similar size; caching them removes
+// hipMalloc/hipFree from the steady state.
+struct DevPool {
+    std::multimap<size_t, void *> free_blocks;
+    std::map<void *, size_t> live;
+    size_t cached_bytes = 0;
+    void *get(size_t bytes);
+    void put(void *p);
+    void release_all();
+};
+
+struct gls_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevPool pool;
+    // library tables resident on the device
+    uint32_t *d_npn = nullptr;          // 65536 x (phase | perm << 5 | cls << 10)
+    uint8_t *d_perms4 = nullptr;        // 24 x 4
+    uint16_t *d_lib_nodes = nullptr;    // packed forest nodes
+    uint16_t *d_lib_node_off = nullptr; // 223
+    uint16_t *d_lib_root_off = nullptr; // 223
+    uint8_t *d_lib_roots = nullptr;
+    hipEvent_t ev[16] = {};
+    bool lib_ready = false;
+};
+
+// A device-resident AIG in the AIGER literal convention.
+struct DevGraph {
+    int32_t nObjs = 0, nPIs = 0, nPOs = 0;
+    uint32_t *f0 = nullptr, *f1 = nullptr, *pos = nullptr;
+    uint32_t *level = nullptr; // optional: node levels (nObjs)
+};
+
+struct gls_aig {
+    gls_ctx *ctx = nullptr;
+    int32_t nPIs = 0, nPOs = 0, nObjs = 0;
+    int32_t levels = -1;
+    bool strashed = false;
+    bool host_valid = false, dev_valid = false;
+    std::vector<uint32_t> h_f0, h_f1, h_pos;  // AIGER convention
+    DevGraph dev;                               // device copy (pool memory)
+    bool dev_level_valid = false;
+    gls_op_info last = {};
+};
+
+// pool helpers
+template <class T> T *pool_get(gls_ctx *ctx, size_t n) {
+    return static_cast<T *>(ctx->pool.get(n * sizeof(T) + 16));
+}
+void pool_put(gls_ctx *ctx, void *p);
+
+// host <-> device
+int aig_ensure_device(gls_aig *aig);
+int aig_ensure_host(gls_aig *aig);
+void aig_drop_device(gls_aig *aig);
+int ctx_ensure_library(gls_ctx *ctx);
+
+// host-side helpers
+void npn_generate(uint8_t *phase, uint8_t *perm, uint8_t *cls, uint8_t *perms4);
+int aiger_parse(const char *buf, int64_t len, std::vector<uint32_t> &f0, std::vector<uint32_t> &f1,
+                std::vector<uint32_t> &pos, int32_t &nPIs, std::string &err);
+int aiger_write(const char *path, int32_t nPIs, const std::vector<uint32_t> &f0,
+                const std::vector<uint32_t> &f1, const std::vector<uint32_t> &pos);
+
+// device passes (strash.hip / rewrite.hip).  Both consume `in` (the caller
+// releases it) and fill `out` with fresh pool memory, out.level included.
+int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, gls_op_info *info);
+int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart);
+int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, int32_t *depth,
+                gls_op_info *info);
+int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
+                       uint32_t *d_cuts, int32_t *d_nref);
+int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,
+                 int useZeros, int32_t *d_best);
+void dev_graph_free(gls_ctx *ctx, DevGraph &g);
+int dev_graph_alloc(gls_ctx *ctx, DevGraph &g, int32_t nObjs, int32_t nPIs, int32_t nPOs);

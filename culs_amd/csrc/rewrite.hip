@@ -1,0 +1,1040 @@
+// DAG-aware rewriting with 4-input cuts on the device.
+//
+// Stages (reference in parentheses, paths relative to the CULS tree):
+//   reorder   level-major renumbering, ties by id (rewrite.cu:1245-1279)
+//   cuts      per-level 4-feasible cut enumeration, 16-bit truth tables, support
+//             minimisation, CULS slot policy (rewrite.cu:49-294)
+//   table     strash table of the current AND nodes (replaces the chained
+//             table of rewrite.cu:328-367, 450-471)
+//   eval      MFFC size within the cut, NPN class, library matching, nodes
+//             added per subgraph root, best (gain, root level) (rewrite.cu:296-447)
+//   plan      the chosen replacement of every candidate: deleted set W (root +
+//             MFFC), leaves L, reused existing nodes M, nodes to create
+//   select    greedy conflict-free selection by priority (gain desc, id asc),
+//             computed in rounds of local maxima with atomicMax claims; it
+//             replaces the reference's sequential host choice pass
+//             (rewrite.cu:811-897), which cannot run in parallel
+//   build     new nodes for the selected plans, ids by prefix sum, fanin
+//             substitution of replaced roots, then the device strash.
+//
+// Everything is exact integer work and independent of thread timing: every
+// shared decision goes through a prefix sum, a min/max atomic or a stable sort.
+#include "gls_internal.h"
+#include "dev_common.h"
+#include "rwlib_forest.h"
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+namespace {
+
+constexpr int NCUT = 8;
+constexpr int MFFC_TAB = 8;
+constexpr int NMATCH = 54;
+constexpr int EVAL_BLOCK = 64;
+constexpr int CUT_BLOCK = 128;
+constexpr int PLAN_MAXM = 16;
+
+struct Lib {
+    const uint32_t *npn;     // phase | perm << 5 | cls << 10
+    const uint8_t *perms4;
+    const uint16_t *nodes, *nodeOff, *rootOff;
+    const uint8_t *roots;
+};
+
+__device__ __forceinline__ int lib_nn(const Lib &L, int c) { return L.nodeOff[c + 1] - L.nodeOff[c]; }
+__device__ __forceinline__ int lib_nr(const Lib &L, int c) { return L.rootOff[c + 1] - L.rootOff[c]; }
+__device__ __forceinline__ int lib_root(const Lib &L, int c, int k) { return L.roots[L.rootOff[c] + k]; }
+__device__ __forceinline__ uint32_t lib_word(const Lib &L, int c, int j) { return L.nodes[L.nodeOff[c] + j]; }
+
+// ------------------------------------------------------------------------
+// cut enumeration
+// ------------------------------------------------------------------------
+struct DCut {
+    uint32_t leaves[4];
+    uint32_t tt, value;
+    int32_t nLeaves, used;
+};
+
+__device__ __forceinline__ uint32_t cut_sign(const DCut &c) {
+    uint32_t s = 0;
+    for (int i = 0; i < 4; i++)
+        if (i < c.nLeaves) s |= 1u << (c.leaves[i] & 31);
+    return s;
+}
+
+__device__ __forceinline__ uint32_t cut_value(const DCut &c, const int32_t *nRef) {
+    int value = 0, nOnes = 0;
+    for (int i = 0; i < 4; i++)
+        if (i < c.nLeaves) {
+            int r = nRef[c.leaves[i]];
+            value += r;
+            nOnes += (r == 1);
+        }
+    if (c.nLeaves < 2) return 1001;
+    if (value > 1000) value = 1000;
+    if (nOnes > 3) value = 5 - nOnes;
+    return (uint32_t)value;
+}
+
+__device__ __forceinline__ void unpack_cut(const uint32_t *p, DCut &c) {
+    c.leaves[0] = p[0]; c.leaves[1] = p[1]; c.leaves[2] = p[2]; c.leaves[3] = p[3];
+    uint32_t m = p[4];
+    c.tt = m & 0xFFFF;
+    c.value = (m >> 16) & 0x7FF;
+    c.nLeaves = (m >> 27) & 7;
+    c.used = (m >> 30) & 1;
+}
+
+__device__ int find_cut(DCut *cs) {
+    for (int i = 0; i < NCUT; i++)
+        if (!cs[i].used) return i;
+    int ans = -1;
+    for (int i = 0; i < NCUT; i++)
+        if (cs[i].nLeaves > 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
+    if (ans == -1)
+        for (int i = 0; i < NCUT; i++)
+            if (cs[i].nLeaves == 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
+    if (ans == -1)
+        for (int i = 0; i < NCUT; i++)
+            if (cs[i].nLeaves < 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
+    cs[ans].used = 0;
+    return ans;
+}
+
+// sorted merge of two leaf sets, a.nLeaves >= b.nLeaves; fails above 4 leaves
+__device__ int merge_ordered(const DCut &a, const DCut &b, DCut &c) {
+    int n = 0, i = 0, k = 0;
+    for (n = 0; n < 4; n++) {
+        if (k == b.nLeaves) {
+            if (i == a.nLeaves) { c.nLeaves = n; return 1; }
+            c.leaves[n] = a.leaves[i++];
+            continue;
+        }
+        if (i == a.nLeaves) { c.leaves[n] = b.leaves[k++]; continue; }
+        if (a.leaves[i] < b.leaves[k]) { c.leaves[n] = a.leaves[i++]; continue; }
+        if (a.leaves[i] > b.leaves[k]) { c.leaves[n] = b.leaves[k++]; continue; }
+        c.leaves[n] = a.leaves[i++];
+        k++;
+    }
+    if (i < a.nLeaves || k < b.nLeaves) return 0;
+    c.nLeaves = n;
+    return 1;
+}
+
+__device__ int dominates(const DCut &a, const DCut &b) {
+    for (int i = 0; i < a.nLeaves; i++) {
+        int ok = 0;
+        for (int j = 0; j < b.nLeaves; j++) ok |= (b.leaves[j] == a.leaves[i]);
+        if (!ok) return 0;
+    }
+    return 1;
+}
+
+__device__ int cut_filter(DCut *cs, int id) {
+    uint32_t sid = cut_sign(cs[id]);
+    for (int i = 0; i < NCUT; i++) {
+        if (i == id || !cs[i].used) continue;
+        uint32_t si = cut_sign(cs[i]);
+        if (cs[i].nLeaves > cs[id].nLeaves) {
+            if ((si & sid) != sid) continue;
+            if (dominates(cs[id], cs[i])) cs[i].used = 0;
+        } else {
+            if ((si & sid) != si) continue;
+            if (dominates(cs[i], cs[id])) { cs[id].used = 0; return 1; }
+        }
+    }
+    return 0;
+}
+
+__device__ __forceinline__ uint32_t swap_adj(uint32_t t, int v) {
+    if (v == 0) return (t & 0x99999999u) | ((t & 0x22222222u) << 1) | ((t & 0x44444444u) >> 1);
+    if (v == 1) return (t & 0xC3C3C3C3u) | ((t & 0x0C0C0C0Cu) << 2) | ((t & 0x30303030u) >> 2);
+    if (v == 2) return (t & 0xF00FF00Fu) | ((t & 0x00F000F0u) << 4) | ((t & 0x0F000F00u) >> 4);
+    return 0;
+}
+
+__device__ uint32_t tt_stretch(uint32_t t, int nVar, int phase) {
+    int var = nVar - 1;
+    for (int i = 3; i >= 0; i--)
+        if ((phase >> i) & 1) {
+            for (int k = var; k < i; k++) t = swap_adj(t, k);
+            var--;
+        }
+    return t;
+}
+
+__device__ int truth_phase(const DCut &x, const DCut &c) {
+    int ph = 0;
+    for (int i = 0; i < c.nLeaves; i++)
+        for (int j = 0; j < x.nLeaves; j++)
+            if (x.leaves[j] == c.leaves[i]) ph |= 1 << i;
+    return ph;
+}
+
+__device__ uint32_t tt_shrink(uint32_t t, int phase) {
+    int var = 0;
+    for (int i = 0; i < 4; i++)
+        if (phase & (1 << i)) {
+            for (int k = i - 1; k >= var; k--) t = swap_adj(t, k);
+            var++;
+        }
+    return t;
+}
+
+__device__ int minimize_support(DCut &c) {
+    const uint32_t m0[4] = {0x5555, 0x3333, 0x0F0F, 0x00FF};
+    const uint32_t m1[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
+    int phase = 0, nL = c.nLeaves;
+    uint32_t t = c.tt & 0xFFFF;
+    for (int i = 0; i < 4; i++) {
+        if (i >= c.nLeaves) break;
+        if ((t & m0[i]) == ((t & m1[i]) >> (1 << i))) nL--;
+        else phase |= 1 << i;
+    }
+    if (nL == c.nLeaves) return 0;
+    t = tt_shrink(t, phase);
+    c.tt = t & 0xFFFF;
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+        if (i < c.nLeaves && ((phase >> i) & 1)) c.leaves[k++] = c.leaves[i];
+    c.nLeaves = nL;
+    return 1;
+}
+
+__device__ void store_cuts(uint32_t *out, const DCut *cs) {
+    for (int s = 0; s < NCUT; s++) {
+        const DCut &c = cs[s];
+        uint32_t *o = out + 5 * s;
+        for (int i = 0; i < 4; i++) o[i] = (c.used && i < c.nLeaves) ? c.leaves[i] : 0u;
+        o[4] = c.used ? ((c.tt & 0xFFFF) | (c.value & 0x7FF) << 16 | (uint32_t)(c.nLeaves & 7) << 27 | 1u << 30) : 0u;
+    }
+}
+
+__global__ void k_cuts_pi(int32_t nPIs, const int32_t *__restrict__ nRef, uint32_t *cuts) {
+    int64_t v = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v > nPIs) return;
+    DCut c[NCUT];
+    for (int s = 0; s < NCUT; s++) { c[s].used = 0; c[s].nLeaves = 0; c[s].tt = 0; c[s].value = 0; }
+    c[0].used = 1; c[0].tt = 0xAAAA; c[0].nLeaves = 1; c[0].leaves[0] = (uint32_t)v;
+    c[0].value = cut_value(c[0], nRef);
+    store_cuts(cuts + (size_t)v * NCUT * 5, c);
+}
+
+// One thread per node of one level; the node's 8 slots live in LDS.
+__global__ void __launch_bounds__(CUT_BLOCK) k_cuts_level(int32_t start, int32_t count,
+                                                          const uint32_t *__restrict__ f0,
+                                                          const uint32_t *__restrict__ f1,
+                                                          const int32_t *__restrict__ nRef, uint32_t *cuts) {
+    __shared__ DCut sh[CUT_BLOCK][NCUT];
+    int i = blockIdx.x * CUT_BLOCK + threadIdx.x;
+    if (i >= count) return;
+    const int idx = start + i;
+    DCut *cs = sh[threadIdx.x];
+    for (int s = 0; s < NCUT; s++) { cs[s].used = 0; cs[s].nLeaves = 0; cs[s].tt = 0; cs[s].value = 0; }
+    cs[0].used = 1; cs[0].tt = 0xAAAA; cs[0].nLeaves = 1; cs[0].leaves[0] = (uint32_t)idx;
+    cs[0].value = cut_value(cs[0], nRef);
+    const uint32_t l0 = f0[idx], l1 = f1[idx];
+    const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
+    const uint32_t *p0 = cuts + (size_t)in0 * NCUT * 5, *p1 = cuts + (size_t)in1 * NCUT * 5;
+    bool done = false;
+    for (int a_i = 0; a_i < NCUT && !done; a_i++) {
+        DCut a;
+        unpack_cut(p0 + 5 * a_i, a);
+        if (!a.used) continue;
+        const uint32_t sa = cut_sign(a);
+        for (int b_j = 0; b_j < NCUT; b_j++) {
+            DCut b;
+            unpack_cut(p1 + 5 * b_j, b);
+            if (!b.used) continue;
+            if (__popc(sa | cut_sign(b)) > 4) continue;
+            int id = find_cut(cs);
+            DCut m = cs[id];
+            if (a.nLeaves >= b.nLeaves) { if (!merge_ordered(a, b, m)) { cs[id] = m; continue; } }
+            else if (!merge_ordered(b, a, m)) { cs[id] = m; continue; }
+            m.used = 1;
+            cs[id] = m;
+            if (cut_filter(cs, id)) continue;
+            uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
+            t0 = tt_stretch(t0, a.nLeaves, truth_phase(a, cs[id]));
+            t1 = tt_stretch(t1, b.nLeaves, truth_phase(b, cs[id]));
+            cs[id].tt = (t0 & t1) & 0xFFFF;
+            DCut mm = cs[id];
+            int shrunk = minimize_support(mm);
+            cs[id] = mm;
+            if (shrunk) cut_filter(cs, id);
+            cs[id].value = cut_value(cs[id], nRef);
+            if (cs[id].nLeaves < 2) { done = true; break; }
+        }
+    }
+    store_cuts(cuts + (size_t)idx * NCUT * 5, cs);
+}
+
+__global__ void k_nref(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                       int32_t nPOs, const uint32_t *__restrict__ pos, int32_t *nRef) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
+        atomicAdd(&nRef[LID(f0[v])], 1);
+        atomicAdd(&nRef[LID(f1[v])], 1);
+    }
+    for (int64_t i = tid; i < nPOs; i += stride) atomicAdd(&nRef[LID(pos[i])], 1);
+}
+
+__global__ void k_table_build(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
+                              const uint32_t *__restrict__ f1, unsigned long long *keys, uint32_t *vals,
+                              uint64_t mask, uint32_t *err) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + tid; v < n; v += stride)
+        ht_insert_unique(keys, vals, mask, and_key(f0[v], f1[v]), (uint32_t)v, err);
+}
+
+// ------------------------------------------------------------------------
+// evaluation
+// ------------------------------------------------------------------------
+struct GraphView {
+    const uint32_t *f0, *f1;
+    const int32_t *nRef;
+    const uint32_t *level;
+    const unsigned long long *htK;
+    const uint32_t *htV;
+    uint64_t mask;
+};
+
+// per-thread column of an LDS array laid out [j][lane]
+struct Col {
+    int32_t *p;
+    __device__ __forceinline__ int32_t &operator[](int j) const { return p[j * EVAL_BLOCK]; }
+};
+
+struct Mffc {
+    int size;
+    int32_t id[MFFC_TAB], num[MFFC_TAB];
+};
+
+__device__ __forceinline__ int mffc_decrease(const GraphView &g, int x, const int *leaves, int nL, Mffc &t) {
+    for (int i = 0; i < 4; i++)
+        if (i < nL && x == leaves[i]) return 1;
+    for (int i = 0; i < MFFC_TAB; i++)
+        if (i < t.size && t.id[i] == x) return --t.num[i];
+    if (t.size == MFFC_TAB) return 1;
+    int r = g.nRef[x] - 1;
+    t.id[t.size] = x;
+    t.num[t.size] = r;
+    t.size++;
+    return r;
+}
+
+// CalcMFFC (rewrite.cu:319-326) without recursion: an explicit DFS stack that
+// visits fanin0's subtree before decreasing fanin1, as the recursion does.
+__device__ int calc_mffc(const GraphView &g, int root, const int *leaves, int nL, Mffc &t) {
+    int stkNode[MFFC_TAB + 2], stkStage[MFFC_TAB + 2];
+    int sp = 1, ans = 1;
+    stkNode[0] = root;
+    stkStage[0] = 0;
+    t.size = 0;
+    while (sp) {
+        int cur = stkNode[sp - 1];
+        int stage = stkStage[sp - 1];
+        if (stage < 2) {
+            stkStage[sp - 1] = stage + 1;
+            int x = (int)LID(stage == 0 ? g.f0[cur] : g.f1[cur]);
+            if (mffc_decrease(g, x, leaves, nL, t) == 0) {
+                stkNode[sp] = x;
+                stkStage[sp] = 0;
+                sp++;
+                ans++;
+            }
+        } else {
+            sp--;
+        }
+    }
+    return ans;
+}
+
+__device__ __forceinline__ int mffc_deleted(const Mffc &t, int x) {
+    for (int i = 0; i < MFFC_TAB; i++)
+        if (i < t.size && t.id[i] == x) return t.num[i] == 0;
+    return 0;
+}
+
+__device__ __forceinline__ int table_lookup(const GraphView &g, int in0, int in1, int c0, int c1) {
+    if (in0 > in1) { int x = in0; in0 = in1; in1 = x; x = c0; c0 = c1; c1 = x; }
+    uint32_t v;
+    if (ht_find(g.htK, g.htV, g.mask, and_key(2u * (uint32_t)in0 + (uint32_t)c0, 2u * (uint32_t)in1 + (uint32_t)c1), &v))
+        return (int)v;
+    return -1;
+}
+
+// library matching of one cut (rewrite.cu:391-418); returns the class
+__device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, const int *leaves,
+                         const Mffc &t, Col match, Col mlev, uint32_t &isC) {
+    uint32_t w = L.npn[tt];
+    int uPhase = w & 31, perm = (w >> 5) & 31, cls = (w >> 10) & 255;
+    const uint8_t *pp = L.perms4 + 4 * perm;
+    for (int j = 0; j < NMATCH; j++) { match[j] = -1; mlev[j] = -1; }
+    isC = 0;
+    for (int j = 0; j < 4; j++) {
+        int leaf = leaves[pp[j]];
+        match[j] = leaf;
+        mlev[j] = (int32_t)g.level[leaf];
+        if ((uPhase >> j) & 1) isC |= 1u << j;
+    }
+    const int nn = lib_nn(L, cls);
+    for (int j = 0; j < nn; j++) {
+        uint32_t wd = lib_word(L, cls, j);
+        int a = wd & 63, b = (wd >> 6) & 63, ca = (wd >> 12) & 1, cb = (wd >> 13) & 1;
+        int num = j + 4;
+        int la = mlev[a], lb = mlev[b];
+        mlev[num] = 1 + (la > lb ? la : lb);
+        int ma = match[a], mb = match[b];
+        if (ma == -1 || mb == -1 || ma == id || mb == id) continue;
+        int nid = table_lookup(g, ma, mb, (int)((isC >> a) & 1) ^ ca, (int)((isC >> b) & 1) ^ cb);
+        if (nid != -1 && !mffc_deleted(t, nid)) {
+            match[num] = nid;
+            mlev[num] = (int32_t)g.level[nid];
+        }
+    }
+    return cls;
+}
+
+__device__ __forceinline__ uint64_t added_set(const Lib &L, int cls, int rt, Col match) {
+    uint64_t used = 1ull << rt, add = 0;
+    for (int j = rt; j >= 4; j--) {
+        if (!((used >> j) & 1) || match[j] > -1) continue;
+        add |= 1ull << j;
+        uint32_t wd = lib_word(L, cls, j - 4);
+        used |= 1ull << (wd & 63);
+        used |= 1ull << ((wd >> 6) & 63);
+    }
+    return add;
+}
+
+__device__ __forceinline__ void cut_leaves4(const uint32_t *p, int *leaves, int &nL, uint32_t &tt) {
+    uint32_t m = p[4];
+    nL = (m >> 27) & 7;
+    tt = m & 0xFFFF;
+    for (int i = 0; i < 4; i++) leaves[i] = (int)p[i];
+    if (nL == 3) { leaves[3] = 0; nL = 4; }
+}
+
+__global__ void __launch_bounds__(EVAL_BLOCK) k_eval(int32_t n, int32_t nPIs, GraphView g, Lib L,
+                                                      const uint32_t *__restrict__ cuts, int useZeros,
+                                                      int4 *best) {
+    __shared__ int32_t s_match[NMATCH * EVAL_BLOCK], s_mlev[NMATCH * EVAL_BLOCK];
+    int64_t v = nPIs + 1 + blockIdx.x * (int64_t)EVAL_BLOCK + threadIdx.x;
+    if (v >= n) return;
+    Col match{s_match + threadIdx.x}, mlev{s_mlev + threadIdx.x};
+    const int id = (int)v;
+    int reduction = -1, bestLevel = 99999999, bestCut = -1, bestOut = 0;
+    const uint32_t *cs = cuts + (size_t)v * NCUT * 5;
+    for (int i = 0; i < NCUT; i++) {
+        const uint32_t *p = cs + 5 * i;
+        uint32_t meta = p[4];
+        if (!((meta >> 30) & 1) || ((meta >> 27) & 7) < 3) continue;
+        int leaves[4], nL;
+        uint32_t tt;
+        cut_leaves4(p, leaves, nL, tt);
+        Mffc t;
+        int saved = calc_mffc(g, id, leaves, nL, t);
+        uint32_t isC;
+        int cls = match_cut(g, L, id, tt, leaves, t, match, mlev, isC);
+        const int nr = lib_nr(L, cls);
+        for (int out = 0; out < nr; out++) {
+            int rt = lib_root(L, cls, out);
+            if (match[rt] == id) continue;
+            int added = __popcll(added_set(L, cls, rt, match));
+            int gain = saved - added;
+            if (gain < 0 || (gain == 0 && !useZeros)) continue;
+            int rl = mlev[rt];
+            if (gain < reduction || (gain == reduction && rl >= bestLevel)) continue;
+            reduction = gain;
+            bestLevel = rl;
+            bestCut = i;
+            bestOut = out;
+        }
+    }
+    best[v] = bestCut >= 0 ? make_int4(bestCut, bestOut, reduction, bestLevel) : make_int4(-1, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------------
+// plans and selection
+// ------------------------------------------------------------------------
+struct Plan {
+    int32_t id, gain;
+    int32_t nW, nL, nM, nNew, valid, pad;
+    int32_t W[MFFC_TAB + 1], L[4], M[PLAN_MAXM];
+};
+
+__device__ uint32_t lib_truth(const Lib &L, int cls, int rt, const uint8_t *pp, uint32_t isC) {
+    const uint32_t ev[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
+    uint32_t tt[64];
+    for (int j = 0; j < 4; j++) tt[j] = (ev[pp[j]] ^ (((isC >> j) & 1) ? 0xFFFFu : 0u)) & 0xFFFF;
+    for (int j = 4; j <= rt; j++) {
+        uint32_t wd = lib_word(L, cls, j - 4);
+        uint32_t a = tt[wd & 63] ^ (((wd >> 12) & 1) ? 0xFFFFu : 0u);
+        uint32_t b = tt[(wd >> 6) & 63] ^ (((wd >> 13) & 1) ? 0xFFFFu : 0u);
+        tt[j] = a & b & 0xFFFF;
+    }
+    return tt[rt];
+}
+
+struct PlanCore {
+    int cls, rt, compl_, valid;
+    uint32_t isC;
+    uint64_t addMask;
+};
+
+// Recomputes the chosen replacement of node `id` exactly as evaluated.
+__device__ PlanCore plan_core(const GraphView &g, const Lib &L, int id, const uint32_t *p, int out, Mffc &t,
+                              Col match, Col mlev) {
+    int leaves[4], nL;
+    uint32_t tt;
+    cut_leaves4(p, leaves, nL, tt);
+    calc_mffc(g, id, leaves, nL, t);
+    PlanCore pc;
+    pc.cls = match_cut(g, L, id, tt, leaves, t, match, mlev, pc.isC);
+    pc.rt = lib_root(L, pc.cls, out);
+    pc.addMask = added_set(L, pc.cls, pc.rt, match);
+    pc.valid = 1;
+    uint32_t w = L.npn[tt];
+    uint32_t rtt = lib_truth(L, pc.cls, pc.rt, L.perms4 + 4 * ((w >> 5) & 31), pc.isC);
+    if (rtt == tt) pc.compl_ = 0;
+    else if (rtt == ((~tt) & 0xFFFF)) pc.compl_ = 1;
+    else { pc.compl_ = 0; pc.valid = 0; }
+    return pc;
+}
+
+__global__ void __launch_bounds__(EVAL_BLOCK) k_plan(int32_t nCand, const uint32_t *__restrict__ candIds,
+                                                      GraphView g, Lib L, const uint32_t *__restrict__ cuts,
+                                                      const int4 *__restrict__ best, Plan *plans) {
+    __shared__ int32_t s_match[NMATCH * EVAL_BLOCK], s_mlev[NMATCH * EVAL_BLOCK];
+    int64_t c = blockIdx.x * (int64_t)EVAL_BLOCK + threadIdx.x;
+    if (c >= nCand) return;
+    Col match{s_match + threadIdx.x}, mlev{s_mlev + threadIdx.x};
+    const int id = (int)candIds[c];
+    const int4 b = best[id];
+    const uint32_t *p = cuts + ((size_t)id * NCUT + b.x) * 5;
+    Mffc t;
+    PlanCore pc = plan_core(g, L, id, p, b.y, t, match, mlev);
+    Plan pl;
+    pl.id = id;
+    pl.gain = b.z;
+    pl.valid = pc.valid;
+    pl.nNew = __popcll(pc.addMask);
+    pl.pad = 0;
+    pl.nW = 0;
+    pl.W[pl.nW++] = id;
+    for (int i = 0; i < MFFC_TAB; i++)
+        if (i < t.size && t.num[i] == 0) pl.W[pl.nW++] = t.id[i];
+    uint32_t meta = p[4];
+    pl.nL = (meta >> 27) & 7;
+    for (int i = 0; i < 4; i++) pl.L[i] = i < pl.nL ? (int32_t)p[i] : 0;
+    uint64_t used = 1ull << pc.rt;
+    for (int j = pc.rt; j >= 4; j--) {
+        if (!((used >> j) & 1)) continue;
+        uint32_t wd = lib_word(L, pc.cls, j - 4);
+        used |= 1ull << (wd & 63);
+        used |= 1ull << ((wd >> 6) & 63);
+    }
+    pl.nM = 0;
+    for (int j = 4; j <= pc.rt; j++) {
+        if (!((used >> j) & 1) || match[j] <= -1) continue;
+        int m = match[j];
+        int dup = 0;
+        for (int q = 0; q < PLAN_MAXM; q++)
+            if (q < pl.nM && pl.M[q] == m) dup = 1;
+        if (m == id) pl.valid = 0;
+        if (!dup) {
+            if (pl.nM < PLAN_MAXM) pl.M[pl.nM++] = m;
+            else pl.valid = 0;
+        }
+    }
+    plans[c] = pl;
+}
+
+enum : uint32_t { MK_W = 1, MK_WNR = 2, MK_L = 4, MK_M = 8 };
+enum : uint8_t { ST_ACTIVE = 0, ST_SELECTED = 1, ST_REJECTED = 2 };
+
+__device__ __forceinline__ uint64_t stamp(uint32_t round, const Plan &p) {
+    return (static_cast<uint64_t>(round) << 48) | (static_cast<uint64_t>(p.gain & 0xFFFF) << 32) |
+           (0xFFFFFFFFu - static_cast<uint32_t>(p.id));
+}
+
+__global__ void k_sel_init(int32_t nCand, const Plan *__restrict__ plans, uint8_t *state) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c < nCand) state[c] = plans[c].valid ? ST_ACTIVE : ST_REJECTED;
+}
+
+struct Claims {
+    unsigned long long *W, *Wnr, *L, *M;
+};
+
+__global__ void k_sel_claim(uint32_t round, int32_t nCand, const Plan *__restrict__ plans,
+                            const uint8_t *__restrict__ state, Claims cl) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nCand || state[c] != ST_ACTIVE) return;
+    const Plan &p = plans[c];
+    unsigned long long s = stamp(round, p);
+    for (int i = 0; i < p.nW; i++) {
+        atomicMax(&cl.W[p.W[i]], s);
+        if (p.W[i] != p.id) atomicMax(&cl.Wnr[p.W[i]], s);
+    }
+    for (int i = 0; i < p.nL; i++) atomicMax(&cl.L[p.L[i]], s);
+    for (int i = 0; i < p.nM; i++) atomicMax(&cl.M[p.M[i]], s);
+}
+
+__global__ void k_sel_decide(uint32_t round, int32_t nCand, const Plan *__restrict__ plans, uint8_t *state,
+                             Claims cl, uint32_t *mk) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nCand || state[c] != ST_ACTIVE) return;
+    const Plan &p = plans[c];
+    unsigned long long s = stamp(round, p);
+    bool win = true;
+    for (int i = 0; i < p.nW && win; i++) {
+        int x = p.W[i];
+        if (cl.W[x] != s || cl.M[x] >= s) win = false;
+        if (x != p.id && cl.L[x] >= s) win = false;
+    }
+    for (int i = 0; i < p.nL && win; i++)
+        if (cl.Wnr[p.L[i]] >= s) win = false;
+    for (int i = 0; i < p.nM && win; i++)
+        if (cl.W[p.M[i]] >= s) win = false;
+    if (!win) return;
+    state[c] = ST_SELECTED;
+    for (int i = 0; i < p.nW; i++) atomicOr(&mk[p.W[i]], MK_W | (p.W[i] != p.id ? MK_WNR : 0u));
+    for (int i = 0; i < p.nL; i++) atomicOr(&mk[p.L[i]], MK_L);
+    for (int i = 0; i < p.nM; i++) atomicOr(&mk[p.M[i]], MK_M);
+}
+
+__global__ void k_sel_elim(int32_t nCand, const Plan *__restrict__ plans, uint8_t *state,
+                           const uint32_t *__restrict__ mk, uint32_t *activeLeft) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nCand || state[c] != ST_ACTIVE) return;
+    const Plan &p = plans[c];
+    bool conf = false;
+    for (int i = 0; i < p.nW && !conf; i++) {
+        uint32_t m = mk[p.W[i]];
+        if (m & (MK_W | MK_M)) conf = true;
+        if (p.W[i] != p.id && (m & MK_L)) conf = true;
+    }
+    for (int i = 0; i < p.nL && !conf; i++)
+        if (mk[p.L[i]] & MK_WNR) conf = true;
+    for (int i = 0; i < p.nM && !conf; i++)
+        if (mk[p.M[i]] & MK_W) conf = true;
+    if (conf) state[c] = ST_REJECTED;
+    else atomicAdd(activeLeft, 1u);
+}
+
+__global__ void k_new_counts(int32_t nCand, const Plan *__restrict__ plans, const uint8_t *__restrict__ state,
+                             uint32_t *cnt, unsigned long long *gainSum) {
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nCand) return;
+    bool s = state[c] == ST_SELECTED;
+    cnt[c] = s ? (uint32_t)plans[c].nNew : 0u;
+    if (s) atomicAdd(gainSum, (unsigned long long)plans[c].gain);
+}
+
+__global__ void k_iota2(int32_t n, uint32_t *repl) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < n; v += stride) repl[v] = 2u * (uint32_t)v;
+}
+
+// Creates the new nodes of each selected plan at ids n + off[c] + k, in
+// ascending library index order, and records the replacement literal.
+__global__ void __launch_bounds__(EVAL_BLOCK) k_build(int32_t n, int32_t nCand, const uint32_t *__restrict__ candIds,
+                                                       GraphView g, Lib L, const uint32_t *__restrict__ cuts,
+                                                       const int4 *__restrict__ best, const uint8_t *__restrict__ state,
+                                                       const uint32_t *__restrict__ off, uint32_t *cf0, uint32_t *cf1,
+                                                       uint32_t *repl, uint32_t *err) {
+    __shared__ int32_t s_match[NMATCH * EVAL_BLOCK], s_mlev[NMATCH * EVAL_BLOCK];
+    int64_t c = blockIdx.x * (int64_t)EVAL_BLOCK + threadIdx.x;
+    if (c >= nCand || state[c] != ST_SELECTED) return;
+    Col match{s_match + threadIdx.x}, mlev{s_mlev + threadIdx.x};
+    const int id = (int)candIds[c];
+    const int4 b = best[id];
+    const uint32_t *p = cuts + ((size_t)id * NCUT + b.x) * 5;
+    Mffc t;
+    PlanCore pc = plan_core(g, L, id, p, b.y, t, match, mlev);
+    uint32_t next = (uint32_t)n + off[c];
+    for (int j = 4; j <= pc.rt; j++) {
+        if (!((pc.addMask >> j) & 1)) continue;
+        uint32_t wd = lib_word(L, pc.cls, j - 4);
+        int a = wd & 63, bb = (wd >> 6) & 63;
+        if (match[a] < 0 || match[bb] < 0) {  // cannot happen for a consistent plan
+            atomicOr(err, 2u);
+            cf0[next] = 0;
+            cf1[next] = 0;
+            match[j] = (int32_t)next;
+            next++;
+            continue;
+        }
+        uint32_t la = 2u * (uint32_t)match[a] + ((((pc.isC >> a) & 1u) ^ ((wd >> 12) & 1u)));
+        uint32_t lb = 2u * (uint32_t)match[bb] + ((((pc.isC >> bb) & 1u) ^ ((wd >> 13) & 1u)));
+        cf0[next] = la;
+        cf1[next] = lb;
+        match[j] = (int32_t)next;
+        next++;
+    }
+    repl[id] = 2u * (uint32_t)match[pc.rt] + (uint32_t)pc.compl_;
+}
+
+__global__ void k_compose(int32_t n, int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
+                          const uint32_t *__restrict__ f1, const uint32_t *__restrict__ repl, uint32_t *cf0,
+                          uint32_t *cf1) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) {
+        if (v <= nPIs) { cf0[v] = 0; cf1[v] = 0; continue; }
+        uint32_t a = v < n ? f0[v] : cf0[v], b = v < n ? f1[v] : cf1[v];
+        uint32_t ia = LID(a), ib = LID(b);
+        cf0[v] = (ia < (uint32_t)n ? repl[ia] : 2u * ia) ^ LC(a);
+        cf1[v] = (ib < (uint32_t)n ? repl[ib] : 2u * ib) ^ LC(b);
+    }
+}
+
+__global__ void k_compose_pos(int32_t nPOs, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ repl,
+                              uint32_t *cpos) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < nPOs) cpos[i] = repl[LID(pos[i])] ^ LC(pos[i]);
+}
+
+__global__ void k_flag_cand(int32_t n, const int4 *__restrict__ best, uint8_t *flag) {
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v < n) flag[v] = best[v].x >= 0;
+}
+
+// reorder: new position of every node, fanins remapped and sorted
+__global__ void k_reorder_ids(int32_t m, const uint32_t *__restrict__ sortedIds, uint32_t *newId) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p == 0) newId[0] = 0;
+    if (p < m) newId[sortedIds[p]] = (uint32_t)(p + 1);
+}
+
+__global__ void k_reorder_emit(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
+                               const uint32_t *__restrict__ f1, const uint32_t *__restrict__ level,
+                               const uint32_t *__restrict__ newId, uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t nv = newId[v];
+    olevel[nv] = level[v];
+    if (v <= nPIs) { of0[nv] = 0; of1[nv] = 0; return; }
+    uint32_t a = 2u * newId[LID(f0[v])] | LC(f0[v]);
+    uint32_t b = 2u * newId[LID(f1[v])] | LC(f1[v]);
+    if (a > b) { uint32_t x = a; a = b; b = x; }
+    of0[nv] = a;
+    of1[nv] = b;
+}
+
+__global__ void k_reorder_pos(int32_t nPOs, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ newId,
+                              uint32_t *opos) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < nPOs) opos[i] = 2u * newId[LID(pos[i])] | LC(pos[i]);
+}
+
+__global__ void k_level_starts(int32_t m, const uint32_t *__restrict__ keys, int32_t *starts) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    if (p == 0 || keys[p] != keys[p - 1]) starts[keys[p]] = (int32_t)(p + 1);
+}
+
+__global__ void k_iota1(int32_t m, uint32_t *x) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) x[i] = (uint32_t)(i + 1);
+}
+
+__global__ void k_gather_level(int32_t m, const uint32_t *__restrict__ level, uint32_t *keys) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) keys[i] = level[i + 1];
+}
+
+struct Tmp {
+    gls_ctx *ctx;
+    std::vector<void *> ptrs;
+    bool ok = true;
+    template <class T> T *get(size_t n) {
+        T *p = pool_get<T>(ctx, n);
+        if (p) ptrs.push_back(p);
+        else ok = false;
+        return p;
+    }
+    ~Tmp() {
+        for (void *p : ptrs) pool_put(ctx, p);
+    }
+};
+
+int blocks(int64_t n, int b) { return (int)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
+
+uint32_t bits_for(uint32_t x) {
+    uint32_t b = 1;
+    while (b < 32 && (x >> b) != 0) ++b;
+    return b;
+}
+
+Lib lib_of(gls_ctx *ctx) {
+    return Lib{ctx->d_npn, ctx->d_perms4, ctx->d_lib_nodes, ctx->d_lib_node_off, ctx->d_lib_root_off, ctx->d_lib_roots};
+}
+
+struct Timer {
+    gls_ctx *ctx;
+    int next = 0;
+    void mark() {
+        if (next < 16) hipEventRecord(ctx->ev[next++], ctx->stream);
+    }
+    double ms(int a, int b) {
+        float x = 0;
+        if (b < next) hipEventElapsedTime(&x, ctx->ev[a], ctx->ev[b]);
+        return x;
+    }
+};
+
+} // namespace
+
+int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart) {
+    const int32_t n = in.nObjs, m = n - 1;
+    hipStream_t st = ctx->stream;
+    Tmp t{ctx};
+    uint32_t *keys = t.get<uint32_t>(m + 1), *keys2 = t.get<uint32_t>(m + 1), *ids = t.get<uint32_t>(m + 1);
+    uint32_t *ids2 = t.get<uint32_t>(m + 1), *newId = t.get<uint32_t>(n);
+    if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: allocation");
+    uint32_t maxLevel = 0;
+    GLS_TRY(dev_graph_alloc(ctx, out, n, in.nPIs, in.nPOs));
+    if (m > 0) {
+        k_gather_level<<<blocks(m, kBlock), kBlock, 0, st>>>(m, in.level, keys);
+        k_iota1<<<blocks(m, kBlock), kBlock, 0, st>>>(m, ids);
+        size_t tb = 0;
+        GLS_HIP(ctx, rocprim::reduce(nullptr, tb, keys, keys2, 0u, (size_t)m, rocprim::maximum<uint32_t>(), st));
+        void *tmp = t.get<uint8_t>(tb);
+        if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: reduce storage");
+        GLS_HIP(ctx, rocprim::reduce(tmp, tb, keys, keys2, 0u, (size_t)m, rocprim::maximum<uint32_t>(), st));
+        GLS_HIP(ctx, hipMemcpyAsync(&maxLevel, keys2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        size_t tb2 = 0;
+        uint32_t eb = bits_for(maxLevel);
+        GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb2, keys, keys2, ids, ids2, (size_t)m, 0u, eb, st));
+        void *tmp2 = t.get<uint8_t>(tb2);
+        if (!tmp2) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: sort storage");
+        GLS_HIP(ctx, rocprim::radix_sort_pairs(tmp2, tb2, keys, keys2, ids, ids2, (size_t)m, 0u, eb, st));
+        k_reorder_ids<<<blocks(m, kBlock), kBlock, 0, st>>>(m, ids2, newId);
+        int32_t *d_starts = t.get<int32_t>(maxLevel + 2);
+        if (!d_starts) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: starts");
+        GLS_HIP(ctx, hipMemsetAsync(d_starts, 0, sizeof(int32_t) * (maxLevel + 2), st));
+        k_level_starts<<<blocks(m, kBlock), kBlock, 0, st>>>(m, keys2, d_starts);
+        levelStart.assign(maxLevel + 2, 0);
+        GLS_HIP(ctx, hipMemcpyAsync(levelStart.data(), d_starts, sizeof(int32_t) * (maxLevel + 1), hipMemcpyDeviceToHost, st));
+        k_reorder_emit<<<blocks(n, kBlock), kBlock, 0, st>>>(n, in.nPIs, in.f0, in.f1, in.level, newId, out.f0,
+                                                              out.f1, out.level);
+        if (in.nPOs > 0)
+            k_reorder_pos<<<blocks(in.nPOs, kBlock), kBlock, 0, st>>>(in.nPOs, in.pos, newId, out.pos);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        levelStart[maxLevel + 1] = n;  // one past the last id
+    } else {
+        GLS_HIP(ctx, hipMemsetAsync(out.f0, 0, sizeof(uint32_t) * n, st));
+        GLS_HIP(ctx, hipMemsetAsync(out.f1, 0, sizeof(uint32_t) * n, st));
+        GLS_HIP(ctx, hipMemsetAsync(out.level, 0, sizeof(uint32_t) * n, st));
+        if (in.nPOs > 0)
+            GLS_HIP(ctx, hipMemcpyAsync(out.pos, in.pos, sizeof(uint32_t) * in.nPOs, hipMemcpyDeviceToDevice, st));
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        levelStart.assign(2, n);
+    }
+    return GLS_OK;
+}
+
+int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
+                       int32_t *d_nref) {
+    hipStream_t st = ctx->stream;
+    const int32_t n = g.nObjs;
+    GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
+    GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
+    k_nref<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f0, g.f1, g.nPOs, g.pos, d_nref);
+    if (g.nPIs > 0) k_cuts_pi<<<blocks(g.nPIs, kBlock), kBlock, 0, st>>>(g.nPIs, d_nref, d_cuts);
+    // levelStart[l] = first id of level l (ids 1.. are level-major); the last
+    // entry is one past the last id.
+    const int L = (int)levelStart.size() - 1;
+    for (int l = 1; l < L; l++) {
+        int32_t s = levelStart[l], e = levelStart[l + 1];
+        if (e <= s) continue;
+        k_cuts_level<<<blocks(e - s, CUT_BLOCK), CUT_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+    }
+    GLS_HIP(ctx, hipGetLastError());
+    return GLS_OK;
+}
+
+static int build_table(gls_ctx *ctx, Tmp &t, const DevGraph &g, GraphView &gv, uint32_t *d_err) {
+    hipStream_t st = ctx->stream;
+    uint64_t cap = 1024;
+    while (cap < 2ull * (uint64_t)g.nObjs) cap <<= 1;
+    unsigned long long *keys = t.get<unsigned long long>(cap);
+    uint32_t *vals = t.get<uint32_t>(cap);
+    if (!keys || !vals) return gls_fail(ctx, GLS_ERR_NOMEM, "table allocation");
+    GLS_HIP(ctx, hipMemsetAsync(keys, 0xFF, sizeof(unsigned long long) * cap, st));
+    k_table_build<<<grid_for(g.nObjs), kBlock, 0, st>>>(g.nObjs, g.nPIs, g.f0, g.f1, keys, vals, cap - 1, d_err);
+    GLS_HIP(ctx, hipGetLastError());
+    gv.htK = keys;
+    gv.htV = vals;
+    gv.mask = cap - 1;
+    return GLS_OK;
+}
+
+int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref, int useZeros,
+                 int32_t *d_best) {
+    GLS_TRY(ctx_ensure_library(ctx));
+    hipStream_t st = ctx->stream;
+    Tmp t{ctx};
+    uint32_t *d_err = t.get<uint32_t>(1);
+    if (!d_err) return gls_fail(ctx, GLS_ERR_NOMEM, "eval");
+    GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
+    GraphView gv{g.f0, g.f1, d_nref, g.level, nullptr, nullptr, 0};
+    GLS_TRY(build_table(ctx, t, g, gv, d_err));
+    GLS_HIP(ctx, hipMemsetAsync(d_best, 0xFF, sizeof(int4) * 1, st));
+    int32_t nAnd = g.nObjs - g.nPIs - 1;
+    // PIs and the constant have no candidate
+    GLS_HIP(ctx, hipMemsetAsync(d_best, 0, sizeof(int4) * (size_t)(g.nPIs + 1), st));
+    std::vector<int4> none(g.nPIs + 1, make_int4(-1, 0, 0, 0));
+    GLS_HIP(ctx, hipMemcpyAsync(d_best, none.data(), sizeof(int4) * none.size(), hipMemcpyHostToDevice, st));
+    if (nAnd > 0)
+        k_eval<<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(g.nObjs, g.nPIs, gv, lib_of(ctx), d_cuts, useZeros,
+                                                                reinterpret_cast<int4 *>(d_best));
+    GLS_HIP(ctx, hipGetLastError());
+    GLS_HIP(ctx, hipStreamSynchronize(st));
+    return GLS_OK;
+}
+
+int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, int32_t *depth, gls_op_info *info) {
+    GLS_TRY(ctx_ensure_library(ctx));
+    hipStream_t st = ctx->stream;
+    Timer tm{ctx};
+    tm.mark(); // 0
+    DevGraph g;
+    std::vector<int32_t> levelStart;
+    GLS_TRY(dev_reorder(ctx, in, g, levelStart));
+    tm.mark(); // 1
+    const int32_t n = g.nObjs, nPIs = g.nPIs, nPOs = g.nPOs;
+    int rc = GLS_OK;
+    {
+        Tmp t{ctx};
+        uint32_t *cuts = t.get<uint32_t>((size_t)n * NCUT * 5);
+        int32_t *nref = t.get<int32_t>(n);
+        int4 *best = t.get<int4>(n);
+        uint8_t *flag = t.get<uint8_t>(n);
+        uint32_t *candIds = t.get<uint32_t>(n), *d_cnt = t.get<uint32_t>(4), *d_err = t.get<uint32_t>(4);
+        uint32_t *repl = t.get<uint32_t>(n);
+        unsigned long long *d_gain = t.get<unsigned long long>(1);
+        if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n); }
+        GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 16, st));
+        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref));
+        tm.mark(); // 2
+        GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};
+        GLS_TRY(build_table(ctx, t, g, gv, d_err));
+        tm.mark(); // 3
+        std::vector<int4> none(nPIs + 1, make_int4(-1, 0, 0, 0));
+        GLS_HIP(ctx, hipMemcpyAsync(best, none.data(), sizeof(int4) * none.size(), hipMemcpyHostToDevice, st));
+        const int32_t nAnd = n - nPIs - 1;
+        const Lib L = lib_of(ctx);
+        if (nAnd > 0)
+            k_eval<<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, nPIs, gv, L, cuts, useZeros, best);
+        GLS_HIP(ctx, hipGetLastError());
+        tm.mark(); // 4
+        // candidates in id order
+        k_flag_cand<<<blocks(n, kBlock), kBlock, 0, st>>>(n, best, flag);
+        size_t tb = 0;
+        rocprim::counting_iterator<uint32_t> ids(0u);
+        GLS_HIP(ctx, rocprim::select(nullptr, tb, ids, flag, candIds, d_cnt, (size_t)n, st));
+        void *tmp = t.get<uint8_t>(tb);
+        if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: select storage");
+        GLS_HIP(ctx, rocprim::select(tmp, tb, ids, flag, candIds, d_cnt, (size_t)n, st));
+        uint32_t nCand = 0;
+        GLS_HIP(ctx, hipMemcpyAsync(&nCand, d_cnt, 4, hipMemcpyDeviceToHost, st));
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        uint32_t herr = 0;
+        GLS_HIP(ctx, hipMemcpy(&herr, d_err, 4, hipMemcpyDeviceToHost));
+        if (herr) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: table overflow"); }
+        uint32_t nSel = 0, rounds = 0, totalNew = 0;
+        unsigned long long estGain = 0;
+        k_iota2<<<grid_for(n), kBlock, 0, st>>>(n, repl);
+        Plan *plans = nullptr;
+        uint8_t *state = nullptr;
+        uint32_t *newOff = nullptr;
+        if (nCand > 0) {
+            plans = t.get<Plan>(nCand);
+            state = t.get<uint8_t>(nCand);
+            uint32_t *mk = t.get<uint32_t>(n), *newCnt = t.get<uint32_t>(nCand + 1);
+            newOff = t.get<uint32_t>(nCand + 1);
+            Claims cl{t.get<unsigned long long>(n), t.get<unsigned long long>(n), t.get<unsigned long long>(n),
+                      t.get<unsigned long long>(n)};
+            if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: plan allocation"); }
+            k_plan<<<blocks(nCand, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>((int32_t)nCand, candIds, gv, L, cuts, best, plans);
+            k_sel_init<<<blocks(nCand, kBlock), kBlock, 0, st>>>((int32_t)nCand, plans, state);
+            GLS_HIP(ctx, hipMemsetAsync(mk, 0, sizeof(uint32_t) * n, st));
+            GLS_HIP(ctx, hipMemsetAsync(cl.W, 0, sizeof(unsigned long long) * n, st));
+            GLS_HIP(ctx, hipMemsetAsync(cl.Wnr, 0, sizeof(unsigned long long) * n, st));
+            GLS_HIP(ctx, hipMemsetAsync(cl.L, 0, sizeof(unsigned long long) * n, st));
+            GLS_HIP(ctx, hipMemsetAsync(cl.M, 0, sizeof(unsigned long long) * n, st));
+            for (uint32_t r = 1;; r++) {
+                GLS_HIP(ctx, hipMemsetAsync(d_cnt, 0, 4, st));
+                k_sel_claim<<<blocks(nCand, kBlock), kBlock, 0, st>>>(r, (int32_t)nCand, plans, state, cl);
+                k_sel_decide<<<blocks(nCand, kBlock), kBlock, 0, st>>>(r, (int32_t)nCand, plans, state, cl, mk);
+                k_sel_elim<<<blocks(nCand, kBlock), kBlock, 0, st>>>((int32_t)nCand, plans, state, mk, d_cnt);
+                GLS_HIP(ctx, hipGetLastError());
+                uint32_t left = 0;
+                GLS_HIP(ctx, hipMemcpyAsync(&left, d_cnt, 4, hipMemcpyDeviceToHost, st));
+                GLS_HIP(ctx, hipStreamSynchronize(st));
+                rounds = r;
+                if (left == 0) break;
+                if (r > 60000) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_INTERNAL, "selection did not converge"); }
+            }
+            GLS_HIP(ctx, hipMemsetAsync(d_gain, 0, 8, st));
+            k_new_counts<<<blocks(nCand, kBlock), kBlock, 0, st>>>((int32_t)nCand, plans, state, newCnt, d_gain);
+            GLS_HIP(ctx, hipMemsetAsync(newCnt + nCand, 0, 4, st));
+            size_t tb2 = 0;
+            GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb2, newCnt, newOff, 0u, (size_t)nCand + 1, rocprim::plus<uint32_t>(), st));
+            void *tmp2 = t.get<uint8_t>(tb2);
+            if (!tmp2) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: scan storage");
+            GLS_HIP(ctx, rocprim::exclusive_scan(tmp2, tb2, newCnt, newOff, 0u, (size_t)nCand + 1, rocprim::plus<uint32_t>(), st));
+            GLS_HIP(ctx, hipMemcpyAsync(&totalNew, newOff + nCand, 4, hipMemcpyDeviceToHost, st));
+            GLS_HIP(ctx, hipMemcpyAsync(&estGain, d_gain, 8, hipMemcpyDeviceToHost, st));
+            // count selected (state == 1)
+            std::vector<uint8_t> hs(nCand);
+            GLS_HIP(ctx, hipMemcpyAsync(hs.data(), state, nCand, hipMemcpyDeviceToHost, st));
+            GLS_HIP(ctx, hipStreamSynchronize(st));
+            for (uint8_t s : hs) nSel += (s == ST_SELECTED);
+        }
+        tm.mark(); // 5
+        const int32_t N = n + (int32_t)totalNew;
+        DevGraph comb;
+        GLS_TRY(dev_graph_alloc(ctx, comb, N, nPIs, nPOs));
+        if (nCand > 0 && totalNew + nSel > 0)
+            k_build<<<blocks(nCand, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, (int32_t)nCand, candIds, gv, L, cuts, best,
+                                                                       state, newOff, comb.f0, comb.f1, repl, d_err);
+        k_compose<<<grid_for(N), kBlock, 0, st>>>(n, N, nPIs, g.f0, g.f1, repl, comb.f0, comb.f1);
+        if (nPOs > 0) k_compose_pos<<<blocks(nPOs, kBlock), kBlock, 0, st>>>(nPOs, g.pos, repl, comb.pos);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_HIP(ctx, hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, st));
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        if (herr) { dev_graph_free(ctx, g); dev_graph_free(ctx, comb); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr); }
+        tm.mark(); // 6
+        dev_graph_free(ctx, g);
+        rc = dev_strash(ctx, comb, out, depth, nullptr);
+        dev_graph_free(ctx, comb);
+        tm.mark(); // 7
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        if (info) {
+            info->reorder_ms = tm.ms(0, 1);
+            info->cuts_ms = tm.ms(1, 2);
+            info->table_ms = tm.ms(2, 3);
+            info->eval_ms = tm.ms(3, 4);
+            info->select_ms = tm.ms(4, 5);
+            info->build_ms = tm.ms(5, 6);
+            info->strash_ms = tm.ms(6, 7);
+            info->total_ms = tm.ms(0, 7);
+            info->candidates = nCand;
+            info->selected = nSel;
+            info->new_nodes = totalNew;
+            info->rounds = rounds;
+            info->levels_enumerated = (int64_t)levelStart.size() - 2;
+            info->est_gain = (int64_t)estGain;
+        }
+    }
+    return rc;
+}

@@ -1,0 +1,139 @@
+/* gls.h — C ABI of the MI355X-native rw / strash library (libgls.so).
+ *
+ * Plain C types only.  Every entry point returns GLS_OK (0) or a negative
+ * error code; gls_last_error() gives the message.  A context owns one HIP
+ * device and one HIP stream; it is not thread-safe, use one context per host
+ * thread.  AIG handles belong to the context that created them.
+ *
+ * Literal convention (lit_convention argument):
+ *   GLS_LIT_AIGER: lit = 2*id + complement, lit 0 = constant FALSE (AIGER);
+ *   GLS_LIT_CULS : same, but lit 0 = constant TRUE, lit 1 = FALSE — the
+ *                  convention of the reference's AIGMan arrays
+ *                  (src/common.h:285-286, AigConst1 = 0).
+ * Arrays are indexed by object id: id 0 = constant, ids 1..nPIs = PIs,
+ * ids nPIs+1 .. nObjs-1 = AND nodes with fanin ids smaller than their own
+ * (topological), nObjs = 1 + nPIs + nAnds.  Entries of fanin arrays for the
+ * constant and the PIs are ignored on input and written as 0 on output.
+ */
+#ifndef GLS_H
+#define GLS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLS_OK 0
+#define GLS_ERR_ARG (-1)      /* invalid argument or malformed graph */
+#define GLS_ERR_HIP (-2)      /* HIP runtime error */
+#define GLS_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define GLS_ERR_FORMAT (-4)   /* AIGER parse error */
+#define GLS_ERR_NODEV (-5)    /* no GPU: context is host-only */
+#define GLS_ERR_IO (-6)       /* file open/read/write failed */
+#define GLS_ERR_INTERNAL (-7) /* an internal consistency check failed */
+
+#define GLS_LIT_AIGER 0
+#define GLS_LIT_CULS 1
+
+typedef struct gls_ctx gls_ctx;
+typedef struct gls_aig gls_aig;
+
+typedef struct {
+    int32_t nPIs, nPOs, nAnds, nObjs;
+    int32_t levels;       /* depth, or -1 if not known */
+    int32_t on_device;    /* 1 if the device copy is authoritative */
+} gls_stats;
+
+/* Timings of the last operation on a handle, in milliseconds (HIP events on
+ * the context stream).  Fields not used by an operation are 0. */
+typedef struct {
+    double total_ms;
+    double reorder_ms, cuts_ms, table_ms, eval_ms, select_ms, build_ms, strash_ms;
+    int64_t candidates, selected, new_nodes, rounds, levels_enumerated;
+    int64_t est_gain;
+} gls_op_info;
+
+/* ---- context ---------------------------------------------------------- */
+/* device >= 0 opens that HIP device; device < 0 gives a host-only context
+ * (AIGER I/O, generators, array conversion; device operations then return
+ * GLS_ERR_NODEV).  Replaces the process-wide state of AIGMan's constructor
+ * (src/aig_manager.cu:21-33) and rw's __managed__ globals (rewrite.cu:24-27). */
+int gls_ctx_create(int device, gls_ctx **out);
+void gls_ctx_destroy(gls_ctx *ctx);
+const char *gls_last_error(const gls_ctx *ctx);
+/* 1 if built for and running on a GPU, else 0 */
+int gls_ctx_has_device(const gls_ctx *ctx);
+
+/* ---- AIG handles ------------------------------------------------------ */
+/* Replaces AIGMan's host arrays + toDevice (src/aig_manager.cu:276-312). */
+int gls_aig_from_arrays(gls_ctx *ctx, int32_t nPIs, int32_t nPOs, int32_t nAnds,
+                        const uint32_t *fanin0, const uint32_t *fanin1, const uint32_t *pos,
+                        int32_t lit_convention, gls_aig **out);
+/* Replaces AIGMan::toHost (src/aig_manager.cu:314-336).  fanin0/fanin1 hold
+ * nObjs entries, pos nPOs entries. */
+int gls_aig_to_arrays(gls_aig *aig, uint32_t *fanin0, uint32_t *fanin1, uint32_t *pos,
+                      int32_t lit_convention);
+/* Replaces AIGMan::printStats counters (src/aig_manager.cu:604-612). */
+int gls_aig_stats(gls_aig *aig, gls_stats *out);
+int gls_aig_copy(gls_aig *src, gls_aig **out);
+void gls_aig_free(gls_aig *aig);
+/* Replaces AIGMan::readFile / readFromMemory (src/aig_manager.cu:80-229):
+ * binary AIGER "aig M I L O A", L must be 0. */
+int gls_aig_read_aiger(gls_ctx *ctx, const char *path, gls_aig **out);
+int gls_aig_read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len, gls_aig **out);
+/* Replaces AIGMan::saveFile (src/aig_manager.cu:231-267). */
+int gls_aig_write_aiger(gls_aig *aig, const char *path);
+
+/* ---- the hot path ----------------------------------------------------- */
+/* Structural hashing + dangling-node removal on the device.  Replaces
+ * Aig::strash (src/aig/strash.cuh:12-14, strash.cu:190-314) behind
+ * AIGMan::strash(fCPU=false) (strash.cu:317).  flags: reserved, pass 0. */
+int gls_strash(gls_aig *aig, int32_t flags);
+
+/* One DAG-aware rewriting pass with 4-input cuts (rw; use_zeros=1 is rw -z /
+ * rwz).  Replaces AIGMan::rewrite (src/aig_manager.cu:632-672) and
+ * rw::CPUSolver::Rewrite (src/algorithms/rewrite.cu:811-938).  The AIG stays on
+ * the device; the result is strashed.  flags: reserved, pass 0. */
+int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
+
+/* Info about the last gls_strash / gls_rewrite on this handle. */
+int gls_aig_last_op(gls_aig *aig, gls_op_info *out);
+
+/* ---- stage-level entry points (parity tests and tools) ----------------- */
+/* Level-major renumbering of a strashed AIG as done before rw
+ * (rw::CPUSolver::Reorder, rewrite.cu:1245-1279), applied in place.
+ * level_out (nObjs entries, may be NULL) receives node levels. */
+int gls_reorder(gls_aig *aig, uint32_t *level_out);
+/* Cut sets of every node of a reordered AIG (GPUSolver::EnumerateAndPreEvaluate
+ * cut stage, rewrite.cu:689-705).  cuts: nObjs*8 records of 5 uint32:
+ * leaves[4], meta = truth | value<<16 | nLeaves<<27 | used<<30. */
+int gls_enumerate_cuts(gls_aig *aig, uint32_t *cuts);
+/* Per-node best replacement of a reordered AIG (EvaluateNode, rewrite.cu:378-447).
+ * best: nObjs*4 int32 {cut slot or -1, subgraph index, gain, root level}. */
+int gls_evaluate(gls_aig *aig, int32_t use_zeros, int32_t *best);
+
+/* ---- NPN tables and library (host) ------------------------------------ */
+/* Closed-form NPN tables (the content read by rw::CPUSolver::ReadLibrary,
+ * rewrite.cu:1281-1307): phase (bit 4 = output complement), perm index into
+ * perms4 (24x4), class 0..221.  Each array 65536 bytes; perms4 96 bytes. */
+int gls_npn_tables(uint8_t *phase, uint8_t *perm, uint8_t *cls, uint8_t *perms4);
+
+/* ---- synthetic inputs (host) ------------------------------------------ */
+/* Seeded random AIG: each AND takes two distinct fanins drawn uniformly from
+ * the PIs and the previous `window` ANDs, complemented with p=0.5; POs are the
+ * last nPOs ANDs. */
+int gls_gen_random(gls_ctx *ctx, int32_t nPIs, int32_t nAnds, int32_t nPOs, int32_t window,
+                   uint64_t seed, gls_aig **out);
+/* Seeded deep random AIG: fanin distance ~ Geometric(p) over the previous
+ * `window` ANDs (re-drawn uniformly over PIs and window when out of range). */
+int gls_gen_geometric(gls_ctx *ctx, int32_t nPIs, int32_t nAnds, int32_t nPOs, int32_t window,
+                      double p, uint64_t seed, gls_aig **out);
+/* n x n unsigned array multiplier (XOR = 3 ANDs, MAJ = 3 ANDs) with a fraction
+ * `redundant` of AND nodes re-expressed redundantly (seeded). */
+int gls_gen_multiplier(gls_ctx *ctx, int32_t nbits, double redundant, uint64_t seed,
+                       gls_aig **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLS_H */

@@ -1,0 +1,67 @@
+/* CPU oracle for the rw/strash hot path — TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library, and only as the checker.  The product (culs_amd / libgls.so)
+ * never links, loads or calls it.
+ *
+ * Graph convention used throughout: AIGER literals (lit = 2*id + compl,
+ * lit 0 = constant FALSE, lit 1 = constant TRUE), node 0 = constant,
+ * ids 1..nPIs = PIs, ids > nPIs = AND nodes.  f0/f1 are indexed by id; entries
+ * for the constant and the PIs are 0.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* NPN tables (closed form).  phase: bits 0..3 input complements, bit 4 output
+ * complement; perm: index into perms4; cls: class 0..221; perms4: 24x4. */
+void orc_npn_tables(uint8_t *phase, uint8_t *perm, uint8_t *cls, uint8_t *perms4);
+
+/* Structural hashing.  Returns nObjs of the result (or <0 on error).  Output
+ * arrays must hold nObjs entries (fanins) / nPOs (pos).  *pDepth = number of
+ * level waves of the dangling-free input. */
+int orc_strash(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+               const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos, int *pDepth);
+
+/* Level-major renumbering (ties by id) of an AIG whose ids are topological. */
+int orc_reorder(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+                const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos,
+                uint32_t *olevel);
+
+/* 4-feasible cut enumeration on a reordered AIG.  cuts: nObjs*8 records of
+ * 5 uint32: leaves[4], meta = tt | value<<16 | nLeaves<<27 | used<<30. */
+int orc_cuts(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+             const uint32_t *pos, uint32_t *cuts);
+
+/* Per-node evaluation.  best: nObjs*4 int32 {slot or -1, subgraph index, gain, root level}. */
+int orc_eval(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+             const uint32_t *pos, const uint32_t *cuts, int useZeros, int32_t *best);
+
+/* One rw pass, this package's pipeline (strashed input -> strashed output).
+ * cap: capacity of of0/of1.  stats[8] (may be NULL): candidates, selected,
+ * new nodes, combined objs, depth, rounds(unused), estimated gain, 0. */
+int orc_rw(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+           const uint32_t *pos, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+           uint32_t *opos, int64_t *stats);
+
+/* One rw pass following the reference's own replacement method (sequential
+ * host choice pass) — used for QoR comparison only. */
+int orc_rw_culs(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+                const uint32_t *pos, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+                uint32_t *opos, int64_t *stats);
+
+/* Equivalence by simulation: exhaustive when nPIs <= 16, else nWords*64 random
+ * patterns from seed.  Returns -1 if all POs agree, else the first differing PO. */
+int orc_sim_cmp(int nObjsA, const uint32_t *a0, const uint32_t *a1, const uint32_t *aPos,
+                int nObjsB, const uint32_t *b0, const uint32_t *b1, const uint32_t *bPos,
+                int nPIs, int nPOs, int nWords, uint64_t seed);
+
+/* Depth (max level) of an AIG with topological ids. */
+int orc_depth(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1);
+
+#ifdef __cplusplus
+}
+#endif
