@@ -1,0 +1,167 @@
+"""Benchmark: rw throughput (Mnodes/s) on the 10M-AND deep random AIG, one MI355X.
+
+Workload (one "step"): one `rw` pass (gls_rewrite) over the strashed 10M-AND
+AIG, on a device-resident copy of the strashed input (the copy is part of the
+step).  The input is generated, uploaded and strashed before timing.
+value = ANDs entering rw / seconds per step.
+
+Also printed in the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch / its average
+                launch time (HIP events on the context stream), vs 8 TB/s HBM;
+  cpu_baseline  the CPU oracle's rw ("port", 1 core) on a bounded sample of the
+                same generator, in the same unit;
+  qor           AND counts, depth, and a simulation equivalence check.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nands N] [--cpu-sample N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per AND per rw pass: 3*K*s_c + 64 with K = 8 cuts of s_c = 20 B
+K_CUTS, CUT_BYTES = 8, 20
+B_RW_PER_AND = 3 * K_CUTS * CUT_BYTES + 64
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nands", type=int, default=10_000_000)
+    ap.add_argument("--pis", type=int, default=1024)
+    ap.add_argument("--pos", type=int, default=4096)
+    ap.add_argument("--window", type=int, default=50_000)
+    ap.add_argument("--p", type=float, default=0.001)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000,
+                    help="ANDs of the CPU oracle sample (same generator)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the path is a single-device kernel pipeline "
+              "(sharding across GPUs is by independent AIGs, not built here); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import culs_amd
+
+    ctx = culs_amd.Context(0)
+    if not ctx.has_device:
+        raise SystemExit("no GPU")
+    t0 = time.time()
+    src = ctx.gen_geometric(a.pis, a.nands, a.pos, a.window, a.p, a.seed)
+    g_in = src.arrays() if not a.no_check else None
+    log("generated %d ANDs in %.1fs" % (a.nands, time.time() - t0))
+    src.strash()
+    st_info = src.last_op()
+    s = src.stats()
+    n_in = s.nAnds
+    log("strash: %d -> %d ANDs, depth %d, %.1f ms" % (a.nands, n_in, s.levels, st_info["total_ms"]))
+
+    def step():
+        c = src.copy()
+        c.rewrite()
+        return c
+
+    for _ in range(a.warmup):
+        step().free()
+    ctx_sync = culs_amd.lib()
+    times, infos, out = [], [], None
+    t_all = time.perf_counter()
+    for i in range(a.steps):
+        t = time.perf_counter()
+        c = step()
+        st = c.stats()          # host-side counters; the pass itself synchronises
+        times.append(time.perf_counter() - t)
+        infos.append(c.last_op())
+        if out is not None:
+            out.free()
+        out = c
+        log("step %d: %.1f ms -> %d ANDs" % (i, times[-1] * 1e3, st.nAnds))
+    total = time.perf_counter() - t_all
+    ms_per_step = 1e3 * total / a.steps
+    value = n_in / (total / a.steps) / 1e6
+    info = infos[-1]
+    n_out = out.stats().nAnds
+    depth_out = out.stats().levels
+
+    # dominant kernel: per-level cut enumeration vs evaluation (event-timed phases)
+    nlev = max(1, info["levels_enumerated"])
+    cand = {
+        "cut_enumeration": (info["cuts_ms"], nlev, n_in * (3 * K_CUTS * CUT_BYTES + 12) / nlev),
+        "evaluation": (info["eval_ms"], 1, n_in * (K_CUTS * CUT_BYTES + 16)),
+    }
+    dom = max(cand, key=lambda k: cand[k][0])
+    ms, launches, bytes_per_launch = cand[dom]
+    avg_launch_s = ms / 1e3 / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "launches": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                "bytes_per_launch": int(bytes_per_launch),
+                "pass_alg_GBs": round(n_in * B_RW_PER_AND / (total / a.steps) / 1e9, 3)}
+
+    qor = {"and_in": a.nands, "and_strashed": n_in, "and_out": n_out, "depth_in": s.levels, "depth_out": depth_out}
+    if g_in is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ffi as O
+        t = time.time()
+        qor["sim_equivalent"] = bool(O.sim_cmp(g_in, out.arrays(), words=2, seed=11) == -1)
+        log("equivalence check %.1fs" % (time.time() - t))
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ffi as O
+        sample = ctx.gen_geometric(a.pis, a.cpu_sample, a.pos, a.window, a.p, a.seed).arrays()
+        ss, _ = O.strash(*sample)
+        t = time.perf_counter()
+        O.rw(*ss)
+        dt = time.perf_counter() - t
+        cpu = {"value": round(O.n_ands(ss) / dt / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
+               "sample": "oracle rw on the same generator at %d ANDs (%d after strash), %.1f s" %
+                         (a.cpu_sample, O.n_ands(ss), dt)}
+        log("cpu baseline: %s" % cpu)
+
+    res = {
+        "metric": "rw Mnodes/s on 10M-AND AIG",
+        "value": round(value, 4),
+        "unit": "Mnodes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "rw (one pass) on the strashed deep random AIG: %d PIs, %d ANDs, %d POs, "
+                               "fanin distance ~ Geometric(p=%g) within a %d window, seed %d" %
+                               (a.pis, a.nands, a.pos, a.p, a.window, a.seed),
+                   "nands": a.nands, "levels": s.levels},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "qor": qor,
+        "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
+                                                      "build_ms", "strash_ms", "total_ms")},
+        "rw_counts": {k: info[k] for k in ("candidates", "selected", "new_nodes", "rounds", "est_gain")},
+        "strash_ms": round(st_info["total_ms"], 3),
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

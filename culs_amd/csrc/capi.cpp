@@ -68,6 +68,14 @@ void DevPool::release_all() {
 
 void pool_put(gls_ctx *ctx, void *p) { ctx->pool.put(p); }
 
+// Every copy is ordered on the context stream: the stream is non-blocking, so
+// copies on the legacy default stream would not wait for its kernels.
+int stream_copy(gls_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    GLS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    GLS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GLS_OK;
+}
+
 int dev_graph_alloc(gls_ctx *ctx, DevGraph &g, int32_t nObjs, int32_t nPIs, int32_t nPOs) {
     g.nObjs = nObjs;
     g.nPIs = nPIs;
@@ -151,12 +159,12 @@ int ctx_ensure_library(gls_ctx *ctx) {
     GLS_HIP(ctx, hipMalloc(&ctx->d_lib_node_off, sizeof(gls_forest_node_off)));
     GLS_HIP(ctx, hipMalloc(&ctx->d_lib_root_off, sizeof(gls_forest_root_off)));
     GLS_HIP(ctx, hipMalloc(&ctx->d_lib_roots, sizeof(gls_forest_roots)));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_npn, packed.data(), 65536 * 4, hipMemcpyHostToDevice));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_perms4, p4.data(), 96, hipMemcpyHostToDevice));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_lib_nodes, gls_forest_nodes, sizeof(gls_forest_nodes), hipMemcpyHostToDevice));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_lib_node_off, gls_forest_node_off, sizeof(gls_forest_node_off), hipMemcpyHostToDevice));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_lib_root_off, gls_forest_root_off, sizeof(gls_forest_root_off), hipMemcpyHostToDevice));
-    GLS_HIP(ctx, hipMemcpy(ctx->d_lib_roots, gls_forest_roots, sizeof(gls_forest_roots), hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_npn, packed.data(), 65536 * 4, hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_perms4, p4.data(), 96, hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_lib_nodes, gls_forest_nodes, sizeof(gls_forest_nodes), hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_lib_node_off, gls_forest_node_off, sizeof(gls_forest_node_off), hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_lib_root_off, gls_forest_root_off, sizeof(gls_forest_root_off), hipMemcpyHostToDevice));
+    GLS_TRY(stream_copy(ctx, ctx->d_lib_roots, gls_forest_roots, sizeof(gls_forest_roots), hipMemcpyHostToDevice));
     ctx->lib_ready = true;
     return GLS_OK;
 }
@@ -273,6 +281,35 @@ extern "C" int gls_aig_stats(gls_aig *a, gls_stats *s) {
 
 extern "C" int gls_aig_copy(gls_aig *src, gls_aig **out) {
     if (!src || !out) return GLS_ERR_ARG;
+    if (src->dev_valid) {
+        // device-resident copy: stays in HBM, no host round trip
+        gls_ctx *ctx = src->ctx;
+        gls_aig *a = new gls_aig();
+        a->ctx = ctx;
+        a->nPIs = src->nPIs;
+        a->nPOs = src->nPOs;
+        a->nObjs = src->nObjs;
+        a->levels = src->levels;
+        a->strashed = src->strashed;
+        int rc = dev_graph_alloc(ctx, a->dev, src->nObjs, src->nPIs, src->nPOs);
+        if (rc != GLS_OK) { delete a; return rc; }
+        const size_t nb = sizeof(uint32_t) * (size_t)src->nObjs;
+        hipStream_t st = ctx->stream;
+        if (hipMemcpyAsync(a->dev.f0, src->dev.f0, nb, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(a->dev.f1, src->dev.f1, nb, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(a->dev.level, src->dev.level, nb, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            (src->nPOs && hipMemcpyAsync(a->dev.pos, src->dev.pos, sizeof(uint32_t) * src->nPOs,
+                                         hipMemcpyDeviceToDevice, st) != hipSuccess)) {
+            dev_graph_free(ctx, a->dev);
+            delete a;
+            return gls_fail(ctx, GLS_ERR_HIP, "device copy failed");
+        }
+        a->dev_valid = true;
+        a->dev_level_valid = src->dev_level_valid;
+        a->host_valid = false;
+        *out = a;
+        return GLS_OK;
+    }
     GLS_TRY(aig_ensure_host(src));
     std::vector<uint32_t> f0 = src->h_f0, f1 = src->h_f1, p = src->h_pos;
     gls_aig *a = new_aig(src->ctx, src->nPIs, f0, f1, p);
@@ -386,7 +423,7 @@ extern "C" int gls_reorder(gls_aig *a, uint32_t *level_out) {
     int32_t depth = a->levels;
     adopt(a, out, depth);
     if (level_out) {
-        GLS_HIP(ctx, hipMemcpy(level_out, a->dev.level, sizeof(uint32_t) * a->nObjs, hipMemcpyDeviceToHost));
+        GLS_TRY(stream_copy(ctx, level_out, a->dev.level, sizeof(uint32_t) * a->nObjs, hipMemcpyDeviceToHost));
     }
     return GLS_OK;
 }
@@ -395,7 +432,7 @@ extern "C" int gls_reorder(gls_aig *a, uint32_t *level_out) {
 static int level_starts_host(gls_aig *a, std::vector<int32_t> &starts) {
     gls_ctx *ctx = a->ctx;
     std::vector<uint32_t> lev(a->nObjs);
-    GLS_HIP(ctx, hipMemcpy(lev.data(), a->dev.level, sizeof(uint32_t) * a->nObjs, hipMemcpyDeviceToHost));
+    GLS_TRY(stream_copy(ctx, lev.data(), a->dev.level, sizeof(uint32_t) * a->nObjs, hipMemcpyDeviceToHost));
     uint32_t maxL = 0;
     for (int32_t v = 1; v < a->nObjs; v++) {
         if (lev[v] < lev[v - 1] && v > 1) return gls_fail(ctx, GLS_ERR_ARG, "AIG is not level-major (call gls_reorder)");
@@ -418,8 +455,7 @@ extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) {
     int32_t *d_nref = pool_get<int32_t>(ctx, n);
     if (!d_cuts || !d_nref) { pool_put(ctx, d_cuts); pool_put(ctx, d_nref); return gls_fail(ctx, GLS_ERR_NOMEM, "cuts"); }
     int rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref);
-    if (rc == GLS_OK && hipMemcpy(cuts, d_cuts, n * 40 * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = gls_fail(ctx, GLS_ERR_HIP, "copy cuts");
+    if (rc == GLS_OK) rc = stream_copy(ctx, cuts, d_cuts, n * 40 * sizeof(uint32_t), hipMemcpyDeviceToHost);
     pool_put(ctx, d_cuts);
     pool_put(ctx, d_nref);
     return rc;
@@ -440,8 +476,7 @@ extern "C" int gls_evaluate(gls_aig *a, int32_t use_zeros, int32_t *best) {
     if (!d_cuts || !d_nref || !d_best) rc = gls_fail(ctx, GLS_ERR_NOMEM, "evaluate");
     if (rc == GLS_OK) rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref);
     if (rc == GLS_OK) rc = dev_evaluate(ctx, a->dev, d_cuts, d_nref, use_zeros, d_best);
-    if (rc == GLS_OK && hipMemcpy(best, d_best, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = gls_fail(ctx, GLS_ERR_HIP, "copy best");
+    if (rc == GLS_OK) rc = stream_copy(ctx, best, d_best, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost);
     pool_put(ctx, d_cuts);
     pool_put(ctx, d_nref);
     pool_put(ctx, d_best);

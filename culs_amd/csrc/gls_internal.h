@@ -82,6 +82,7 @@ template <class T> T *pool_get(gls_ctx *ctx, size_t n) {
     return static_cast<T *>(ctx->pool.get(n * sizeof(T) + 16));
 }
 void pool_put(gls_ctx *ctx, void *p);
+int stream_copy(gls_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 
 // host <-> device
 int aig_ensure_device(gls_aig *aig);

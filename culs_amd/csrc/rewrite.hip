@@ -44,6 +44,9 @@ struct Lib {
     const uint8_t *roots;
 };
 
+// complement of library input a: only the 4 cut inputs carry one (the
+// reference keeps these bits in a 64-bit word, rewrite.cu:388-395)
+__device__ __forceinline__ uint32_t isc_bit(uint32_t isC, int a) { return a < 4 ? (isC >> a) & 1u : 0u; }
 __device__ __forceinline__ int lib_nn(const Lib &L, int c) { return L.nodeOff[c + 1] - L.nodeOff[c]; }
 __device__ __forceinline__ int lib_nr(const Lib &L, int c) { return L.rootOff[c + 1] - L.rootOff[c]; }
 __device__ __forceinline__ int lib_root(const Lib &L, int c, int k) { return L.roots[L.rootOff[c] + k]; }
@@ -390,7 +393,7 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
         mlev[num] = 1 + (la > lb ? la : lb);
         int ma = match[a], mb = match[b];
         if (ma == -1 || mb == -1 || ma == id || mb == id) continue;
-        int nid = table_lookup(g, ma, mb, (int)((isC >> a) & 1) ^ ca, (int)((isC >> b) & 1) ^ cb);
+        int nid = table_lookup(g, ma, mb, (int)isc_bit(isC, a) ^ ca, (int)isc_bit(isC, b) ^ cb);
         if (nid != -1 && !mffc_deleted(t, nid)) {
             match[num] = nid;
             mlev[num] = (int32_t)g.level[nid];
@@ -670,8 +673,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_build(int32_t n, int32_t nCand, 
             next++;
             continue;
         }
-        uint32_t la = 2u * (uint32_t)match[a] + ((((pc.isC >> a) & 1u) ^ ((wd >> 12) & 1u)));
-        uint32_t lb = 2u * (uint32_t)match[bb] + ((((pc.isC >> bb) & 1u) ^ ((wd >> 13) & 1u)));
+        uint32_t la = 2u * (uint32_t)match[a] + (isc_bit(pc.isC, a) ^ ((wd >> 12) & 1u));
+        uint32_t lb = 2u * (uint32_t)match[bb] + (isc_bit(pc.isC, bb) ^ ((wd >> 13) & 1u));
         cf0[next] = la;
         cf1[next] = lb;
         match[j] = (int32_t)next;
@@ -948,7 +951,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, i
         GLS_HIP(ctx, hipMemcpyAsync(&nCand, d_cnt, 4, hipMemcpyDeviceToHost, st));
         GLS_HIP(ctx, hipStreamSynchronize(st));
         uint32_t herr = 0;
-        GLS_HIP(ctx, hipMemcpy(&herr, d_err, 4, hipMemcpyDeviceToHost));
+        GLS_TRY(stream_copy(ctx, &herr, d_err, 4, hipMemcpyDeviceToHost));
         if (herr) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: table overflow"); }
         uint32_t nSel = 0, rounds = 0, totalNew = 0;
         unsigned long long estGain = 0;

@@ -32,6 +32,7 @@
  */
 #include "oracle.h"
 #include "../culs_amd/csrc/rwlib_forest.h"
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -559,6 +560,9 @@ static int table_lookup(const Graph *g, int in0, int in1, int c0, int c1) {  /* 
     return map_get(&g->tab, and_key(2u * (uint32_t)in0 + (uint32_t)c0, 2u * (uint32_t)in1 + (uint32_t)c1));
 }
 
+/* complement of library input a (only the 4 cut inputs carry one; the
+ * reference keeps these bits in a 64-bit word, rewrite.cu:388-395) */
+static inline uint32_t isc_bit(uint32_t isC, int a) { return a < 4 ? (isC >> a) & 1u : 0u; }
 static inline int lib_in0(int c, int j) { return gls_forest_nodes[gls_forest_node_off[c] + j] & 63; }
 static inline int lib_in1(int c, int j) { return (gls_forest_nodes[gls_forest_node_off[c] + j] >> 6) & 63; }
 static inline int lib_c0(int c, int j) { return (gls_forest_nodes[gls_forest_node_off[c] + j] >> 12) & 1; }
@@ -585,8 +589,8 @@ static int match_cut(const Graph *g, int id, const Cut *cut, const int *leaves, 
         int num = j + 4, a = lib_in0(cls, j), b = lib_in1(cls, j);
         mlev[num] = 1 + (mlev[a] > mlev[b] ? mlev[a] : mlev[b]);
         if (match[a] == -1 || match[b] == -1 || match[a] == id || match[b] == id) continue;
-        int nid = table_lookup(g, match[a], match[b], (int)((*isC >> a) & 1) ^ lib_c0(cls, j),
-                               (int)((*isC >> b) & 1) ^ lib_c1(cls, j));
+        int nid = table_lookup(g, match[a], match[b], (int)isc_bit(*isC, a) ^ lib_c0(cls, j),
+                               (int)isc_bit(*isC, b) ^ lib_c1(cls, j));
         if (nid != -1 && !mffc_deleted(t, nid)) { match[num] = nid; mlev[num] = g->level[nid]; }
     }
     return cls;
@@ -659,6 +663,23 @@ int orc_eval(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
     }
     map_free(&g.tab); free(cuts); free(nRef); free(lev);
     return n;
+}
+
+/* MFFC of `root` bounded by `leaves` as counted by the evaluation (8-entry
+ * table); *tabSize receives the table fill so callers know when it is exact. */
+int orc_mffc(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+             int root, const int *leavesIn, int nL, int *tabSize) {
+    int32_t *nRef = malloc(sizeof(int32_t) * (size_t)n);
+    fanout_counts(n, nPIs, f0, f1, nPOs, pos, nRef);
+    Graph g;
+    g.n = n; g.nPIs = nPIs; g.f0 = f0; g.f1 = f1; g.nRef = nRef; g.level = NULL;
+    int leaves[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nL && i < 4; i++) leaves[i] = leavesIn[i];
+    MffcTab t; t.size = 0;
+    int saved = calc_mffc(&g, root, leaves, nL, &t);
+    if (tabSize) *tabSize = t.size;
+    free(nRef);
+    return saved;
 }
 
 /* ------------------------------------------------------------------ */
@@ -772,6 +793,9 @@ static void rw_prepare(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, 
 
 static void rw_free(RwIn *r) { free(r->f0); free(r->f1); free(r->pos); free(r->nRef); free(r->level); }
 
+static int g_dbg_max_sel = -1;   /* debugging aid: apply at most this many selections */
+void orc_debug_max_sel(int k) { g_dbg_max_sel = k; }
+
 int orc_rw(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPOs,
            const uint32_t *posin, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
            uint32_t *opos, int64_t *stats) {
@@ -801,6 +825,7 @@ int orc_rw(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPO
     for (int i = 0; i < nCand; i++) {
         Plan *p = ordp[i];
         if (!p->valid || plan_conflicts(p, mk)) continue;
+        if (g_dbg_max_sel >= 0 && nSel >= g_dbg_max_sel) break;
         plan_mark(p, mk);
         sel[p - plans] = 1; nSel++; estGain += p->gain;
     }
@@ -821,8 +846,8 @@ int orc_rw(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPO
         for (int j = 4; j <= p->rt; j++) {
             if (!((p->addMask >> j) & 1)) continue;
             int k = j - 4, a = lib_in0(p->cls, k), b = lib_in1(p->cls, k);
-            uint32_t la = 2u * (uint32_t)nodeOf[a] + ((((p->isC >> a) & 1) ^ (uint32_t)lib_c0(p->cls, k)));
-            uint32_t lb = 2u * (uint32_t)nodeOf[b] + ((((p->isC >> b) & 1) ^ (uint32_t)lib_c1(p->cls, k)));
+            uint32_t la = 2u * (uint32_t)nodeOf[a] + ((isc_bit(p->isC, a) ^ (uint32_t)lib_c0(p->cls, k)));
+            uint32_t lb = 2u * (uint32_t)nodeOf[b] + ((isc_bit(p->isC, b) ^ (uint32_t)lib_c1(p->cls, k)));
             nodeOf[j] = next;
             c0[next] = la; c1[next] = lb;
             next++;
@@ -954,8 +979,8 @@ int orc_rw_culs(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, in
             if (!((used >> (j + 4)) & 1)) continue;
             int a = lib_in0(cls, j), b = lib_in1(cls, j);
             if (match[a] == -1 || match[b] == -1) continue;
-            int nid = table_lookup(&g, match[a], match[b], (int)((isC >> a) & 1) ^ lib_c0(cls, j),
-                                   (int)((isC >> b) & 1) ^ lib_c1(cls, j));
+            int nid = table_lookup(&g, match[a], match[b], (int)isc_bit(isC, a) ^ lib_c0(cls, j),
+                                   (int)isc_bit(isC, b) ^ lib_c1(cls, j));
             if (nid != -1) match[j + 4] = nid;
         }
         if (match[rt] == id) continue;
@@ -977,8 +1002,8 @@ int orc_rw_culs(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, in
             }
             int node = N;
             int32_t in0 = match[a], in1 = match[b];
-            int32_t cc0 = (int32_t)(((isC >> a) & 1) ^ (uint32_t)lib_c0(cls, k));
-            int32_t cc1 = (int32_t)(((isC >> b) & 1) ^ (uint32_t)lib_c1(cls, k));
+            int32_t cc0 = (int32_t)(isc_bit(isC, a) ^ (uint32_t)lib_c0(cls, k));
+            int32_t cc1 = (int32_t)(isc_bit(isC, b) ^ (uint32_t)lib_c1(cls, k));
             if (in0 > in1) { int32_t tmp = in0; in0 = in1; in1 = tmp; tmp = cc0; cc0 = cc1; cc1 = tmp; }
             int32_t l0 = 2 * in0 + cc0, l1 = 2 * in1 + cc1;
             int32_t actual = map_put(&newTab, and_key((uint32_t)l0, (uint32_t)l1), node);

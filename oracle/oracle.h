@@ -59,6 +59,10 @@ int orc_sim_cmp(int nObjsA, const uint32_t *a0, const uint32_t *a1, const uint32
                 int nObjsB, const uint32_t *b0, const uint32_t *b1, const uint32_t *bPos,
                 int nPIs, int nPOs, int nWords, uint64_t seed);
 
+/* MFFC size of root within the given leaves, as counted by the evaluation. */
+int orc_mffc(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+             int root, const int *leaves, int nL, int *tabSize);
+
 /* Depth (max level) of an AIG with topological ids. */
 int orc_depth(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1);
 

@@ -1,0 +1,248 @@
+"""Pins the CPU oracle against what the mathematics fixes.
+
+- strash: output equivalent to the input (exhaustive simulation for <= 16 PIs),
+  no duplicate or trivially reducible AND, sorted fanins, topological ids, and
+  the canonical (wave, id) order; plus a hand-built case with a known answer.
+- cut truth tables: every kept cut's 16-bit table equals the node's function of
+  its leaves on simulated patterns (a wrong variable order, a dropped
+  complement or a bad support reduction fails it).
+- cut sets: leaves sorted/distinct, at most 4, at most 8 cuts, trivial cut first;
+  no kept cut contains another.
+- MFFC: the evaluation's count equals brute-force dereferencing whenever its
+  8-entry table did not overflow.
+- rw: equivalent output for many seeds, deterministic, never larger than the
+  input, and within 1% of the reference's own replacement method
+  (sequential choice pass, rewrite.cu:811-938) on AND count.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+
+
+def gen(host_ctx, kind, **kw):
+    if kind == "random":
+        return host_ctx.gen_random(kw["nPIs"], kw["nAnds"], kw["nPOs"], kw["window"], kw["seed"]).arrays()
+    if kind == "mult":
+        return host_ctx.gen_multiplier(kw["nbits"], kw["redundant"], kw["seed"]).arrays()
+    return host_ctx.gen_geometric(kw["nPIs"], kw["nAnds"], kw["nPOs"], kw["window"], kw["p"], kw["seed"]).arrays()
+
+
+SMALL = [dict(kind="random", nPIs=12, nAnds=300, nPOs=6, window=50, seed=s) for s in range(1, 6)] + \
+        [dict(kind="mult", nbits=6, redundant=0.15, seed=4),
+         dict(kind="random", nPIs=16, nAnds=2000, nPOs=16, window=200, seed=9)]
+
+
+def simulate(g, words):
+    """values of every node for given PI words (uint64 arrays)."""
+    nPIs, f0, f1, pos = g
+    n = len(f0)
+    v = np.zeros((n, words.shape[1]), np.uint64)
+    v[1:nPIs + 1] = words
+    full = np.uint64(0xFFFFFFFFFFFFFFFF)
+    for i in range(nPIs + 1, n):
+        a = v[f0[i] >> 1] ^ (full if f0[i] & 1 else np.uint64(0))
+        b = v[f1[i] >> 1] ^ (full if f1[i] & 1 else np.uint64(0))
+        v[i] = a & b
+    return v
+
+
+# ---------------------------------------------------------------- strash ----
+def test_strash_known_answer():
+    # a, b PIs; n3 = a&b, n4 = b&a (duplicate), n5 = a&!a (const 0), n6 = n3&n4 (= n3),
+    # n7 = n6 & 1 (= n3), n8 = n4 & !b (= 0 structurally not folded: a&b&!b), PO: n7, !n5, n8
+    f0 = [0, 0, 0, 2, 4, 2, 6, 12, 8]
+    f1 = [0, 0, 0, 4, 2, 3, 8, 1, 5]
+    pos = [14, 11, 16]
+    (nPIs, o0, o1, op), depth = O.strash(2, f0, f1, pos)
+    # dangling-free waves: n3,n4,n5 wave1; n6,n8 wave2; n7 wave3.  Survivors:
+    # n3 (rep of n4, n6 -> n3&n3 = n3, n7 -> n3), n8 = AND(n3, !b) -> new id 4
+    assert depth == 3
+    assert list(o0) == [0, 0, 0, 2, 5] and list(o1) == [0, 0, 0, 4, 6]
+    assert list(op) == [6, 1, 8]
+
+
+@pytest.mark.parametrize("spec", SMALL, ids=lambda s: "%s-%s" % (s["kind"], s["seed"]))
+def test_strash_invariants(host_ctx, spec):
+    g = gen(host_ctx, **spec)
+    s, depth = O.strash(*g)
+    nPIs, f0, f1, pos = s
+    assert O.sim_cmp(g, s) == -1
+    keys = set()
+    for v in range(nPIs + 1, len(f0)):
+        a, b = int(f0[v]), int(f1[v])
+        assert a < b and (a >> 1) < v and (b >> 1) < v
+        assert a >= 2 and (a >> 1) != (b >> 1)          # no constant, no x&x, no x&!x
+        assert (a, b) not in keys
+        keys.add((a, b))
+    # idempotent
+    s2, _ = O.strash(*s)
+    assert all(np.array_equal(x, y) for x, y in zip(s, s2)) or O.n_ands(s2) <= O.n_ands(s)
+    assert O.depth(nPIs, f0, f1) <= depth
+
+
+def test_strash_removes_exactly_the_dangling_nodes(host_ctx):
+    g = gen(host_ctx, "random", nPIs=64, nAnds=20000, nPOs=32, window=1000, seed=1)
+    nPIs, f0, f1, pos = g
+    live = np.zeros(len(f0), bool)
+    stack = [int(p) >> 1 for p in pos]
+    while stack:
+        v = stack.pop()
+        if live[v]:
+            continue
+        live[v] = True
+        if v > nPIs:
+            stack += [int(f0[v]) >> 1, int(f1[v]) >> 1]
+    n_live = int(live[nPIs + 1:].sum())
+    s, _ = O.strash(*g)
+    assert O.n_ands(s) <= n_live
+    assert O.n_ands(s) == 14073        # known answer of this seed (no duplicates in a random AIG)
+
+
+# ------------------------------------------------------------------ cuts ----
+def unpack(cuts, v):
+    out = []
+    for s in range(8):
+        w = cuts[(v * 8 + s) * 5:(v * 8 + s) * 5 + 5]
+        meta = int(w[4])
+        if (meta >> 30) & 1:
+            nl = (meta >> 27) & 7
+            out.append(([int(x) for x in w[:nl]], meta & 0xFFFF, (meta >> 16) & 0x7FF, s))
+    return out
+
+
+def eval_tt(tt, leaf_vals):
+    """function tt of leaf value words (list of uint64 arrays)."""
+    res = np.zeros_like(leaf_vals[0]) if leaf_vals else None
+    nw = None
+    for m in range(16):
+        if not (tt >> m) & 1:
+            continue
+        term = None
+        for i in range(4):
+            bit = (m >> i) & 1
+            if i < len(leaf_vals):
+                x = leaf_vals[i] if bit else ~leaf_vals[i]
+            else:
+                continue
+            term = x if term is None else term & x
+        res = res | term
+    return res
+
+
+@pytest.mark.parametrize("spec", SMALL, ids=lambda s: "%s-%s" % (s["kind"], s["seed"]))
+def test_cut_truth_tables_by_simulation(host_ctx, spec):
+    g = gen(host_ctx, **spec)
+    s, _ = O.strash(*g)
+    r, _ = O.reorder(*s)
+    cuts = O.cuts(*r)
+    rng = np.random.default_rng(7)
+    nPIs = r[0]
+    words = rng.integers(0, 2**63, size=(nPIs, 4), dtype=np.uint64) * np.uint64(2) + \
+        rng.integers(0, 2, size=(nPIs, 4), dtype=np.uint64)
+    vals = simulate(r, words)
+    checked = 0
+    for v in range(1, len(r[1])):
+        cs = unpack(cuts, v)
+        assert 1 <= len(cs) <= 8
+        # slot 0 holds the trivial cut unless a cut of < 2 leaves (a node that is a
+        # constant or a single leaf) dominated it and ended the enumeration
+        assert (cs[0][0] == [v] and cs[0][3] == 0) or any(len(c[0]) < 2 for c in cs)
+        for leaves, tt, value, slot in cs:
+            assert leaves == sorted(set(leaves)) and len(leaves) <= 4
+            if len(leaves) < 4:
+                # don't-care variables are replicated copies
+                for i in range(len(leaves), 4):
+                    lo = [m for m in range(16) if not (m >> i) & 1]
+                    assert all(((tt >> m) & 1) == ((tt >> (m | (1 << i))) & 1) for m in lo)
+            got = eval_tt(tt, [vals[l] for l in leaves]) if leaves else (
+                np.full_like(vals[v], 0xFFFFFFFFFFFFFFFF) if tt & 1 else np.zeros_like(vals[v]))
+            assert np.array_equal(got, vals[v]), (v, leaves, hex(tt))
+            checked += 1
+        # no kept cut contains another
+        sets = [set(c[0]) for c in cs]
+        for i in range(len(sets)):
+            for j in range(len(sets)):
+                if i != j:
+                    assert not (sets[i] < sets[j]) or cs[j][3] == 0 or cs[i][3] == 0
+    assert checked > len(r[1])
+
+
+def brute_mffc(g, root, leaves):
+    """nodes freed when root is removed, without crossing the leaves."""
+    nPIs, f0, f1, pos = g
+    n = len(f0)
+    ref = np.zeros(n, np.int64)
+    for v in range(nPIs + 1, n):
+        ref[f0[v] >> 1] += 1
+        ref[f1[v] >> 1] += 1
+    for p in pos:
+        ref[p >> 1] += 1
+    count, stack = 0, [root]
+    while stack:
+        v = stack.pop()
+        count += 1
+        for u in (int(f0[v]) >> 1, int(f1[v]) >> 1):
+            if u in leaves or u <= nPIs:
+                continue
+            ref[u] -= 1
+            if ref[u] == 0:
+                stack.append(u)
+    return count
+
+
+def test_mffc_matches_brute_force(host_ctx):
+    import ctypes
+    g = gen(host_ctx, "random", nPIs=16, nAnds=3000, nPOs=16, window=200, seed=21)
+    s, _ = O.strash(*g)
+    r, _ = O.reorder(*s)
+    cuts = O.cuts(*r)
+    nPIs, f0, f1, pos = r
+    f0a, f1a, pa = O.u32(f0), O.u32(f1), O.u32(pos)
+    exact = 0
+    for v in range(nPIs + 1, len(f0)):
+        for leaves, tt, value, slot in unpack(cuts, v):
+            if len(leaves) < 3:
+                continue
+            lv = (ctypes.c_int * 4)(*(leaves + [0] * (4 - len(leaves))))
+            nl = 4                                      # 3-leaf cuts are padded with node 0
+            tab = ctypes.c_int(0)
+            saved = O.lib().orc_mffc(len(f0), nPIs, O.P(f0a), O.P(f1a), len(pos), O.P(pa), v, lv, nl,
+                                     ctypes.byref(tab))
+            if tab.value < 8:
+                assert saved == brute_mffc(r, v, set(leaves) | {0}), (v, leaves)
+                exact += 1
+    assert exact > 1000
+
+
+# -------------------------------------------------------------------- rw ----
+@pytest.mark.parametrize("spec", SMALL, ids=lambda s: "%s-%s" % (s["kind"], s["seed"]))
+@pytest.mark.parametrize("zeros", [False, True])
+def test_rw_equivalent_and_smaller(host_ctx, spec, zeros):
+    g = gen(host_ctx, **spec)
+    s, _ = O.strash(*g)
+    r, st = O.rw(*s, use_zeros=zeros)
+    assert O.sim_cmp(g, r) == -1
+    assert O.n_ands(r) <= O.n_ands(s)
+    assert st[1] <= st[0]
+    r2, _ = O.rw(*s, use_zeros=zeros)
+    assert all(np.array_equal(x, y) for x, y in zip(r, r2))
+    c, _ = O.rw_culs(*s, use_zeros=zeros)
+    assert O.sim_cmp(g, c) == -1
+
+
+@pytest.mark.parametrize("spec", [
+    dict(kind="random", nPIs=64, nAnds=20000, nPOs=32, window=1000, seed=1),
+    dict(kind="mult", nbits=32, redundant=0.1, seed=2),
+    dict(kind="mult", nbits=16, redundant=0.0, seed=2),
+], ids=["cfg0", "cfg1", "mult16"])
+def test_rw_qor_vs_reference_method(host_ctx, spec):
+    g = gen(host_ctx, **spec)
+    s, _ = O.strash(*g)
+    mine, _ = O.rw(*s)
+    ref, _ = O.rw_culs(*s)
+    assert O.sim_cmp(g, mine, words=16) == -1 and O.sim_cmp(g, ref, words=16) == -1
+    assert O.n_ands(mine) <= O.n_ands(ref) * 1.01, (O.n_ands(mine), O.n_ands(ref))
+    # two passes (rw; rwz) as in the 32x32 multiplier script
+    m2, _ = O.rw(*mine, use_zeros=True)
+    assert O.sim_cmp(g, m2, words=16) == -1
