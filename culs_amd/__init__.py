@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgls.so")
 
 GLS_LIT_AIGER = 0
 GLS_LIT_CULS = 1
+GLS_RW_PERLEVEL_CUTS = 1
 
 
 class GlsError(RuntimeError):
@@ -205,8 +206,12 @@ class Aig:
         self.ctx.check(lib().gls_strash(self._h, 0), "gls_strash")
         return self
 
-    def rewrite(self, use_zeros=False):
-        self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, 0), "gls_rewrite")
+    def rewrite(self, use_zeros=False, per_level_cuts=False):
+        """One rw pass (rw -z / rwz with use_zeros).  per_level_cuts selects one
+        kernel launch per level for cut enumeration instead of one cooperative
+        launch; the results are identical."""
+        flags = GLS_RW_PERLEVEL_CUTS if per_level_cuts else 0
+        self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
         return self
 
     def last_op(self):

@@ -398,11 +398,10 @@ extern "C" int gls_rewrite(gls_aig *a, int32_t use_zeros, int32_t flags) {
     DevGraph out;
     int32_t depth = 0;
     gls_op_info info = {};
-    GLS_TRY(dev_rewrite(ctx, a->dev, use_zeros ? 1 : 0, out, &depth, &info));
+    GLS_TRY(dev_rewrite(ctx, a->dev, use_zeros ? 1 : 0, flags, out, &depth, &info));
     adopt(a, out, depth);
     a->strashed = true;
     a->last = info;
-    (void)flags;
     return GLS_OK;
 }
 
@@ -454,7 +453,7 @@ extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) {
     uint32_t *d_cuts = pool_get<uint32_t>(ctx, n * 40);
     int32_t *d_nref = pool_get<int32_t>(ctx, n);
     if (!d_cuts || !d_nref) { pool_put(ctx, d_cuts); pool_put(ctx, d_nref); return gls_fail(ctx, GLS_ERR_NOMEM, "cuts"); }
-    int rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref);
+    int rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref, 0);
     if (rc == GLS_OK) rc = stream_copy(ctx, cuts, d_cuts, n * 40 * sizeof(uint32_t), hipMemcpyDeviceToHost);
     pool_put(ctx, d_cuts);
     pool_put(ctx, d_nref);
@@ -474,7 +473,7 @@ extern "C" int gls_evaluate(gls_aig *a, int32_t use_zeros, int32_t *best) {
     int32_t *d_best = pool_get<int32_t>(ctx, n * 4);
     int rc = GLS_OK;
     if (!d_cuts || !d_nref || !d_best) rc = gls_fail(ctx, GLS_ERR_NOMEM, "evaluate");
-    if (rc == GLS_OK) rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref);
+    if (rc == GLS_OK) rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref, 0);
     if (rc == GLS_OK) rc = dev_evaluate(ctx, a->dev, d_cuts, d_nref, use_zeros, d_best);
     if (rc == GLS_OK) rc = stream_copy(ctx, best, d_best, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost);
     pool_put(ctx, d_cuts);

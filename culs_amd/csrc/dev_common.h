@@ -26,21 +26,21 @@ __device__ __forceinline__ uint64_t and_key(uint32_t l0, uint32_t l1) {
 }
 
 // Insert `key` with value `val`, keeping the minimum value over all inserts of
-// the same key (values must start at ~0).  Probing is bounded by the capacity;
-// a full table sets *err and returns.
-__device__ __forceinline__ void ht_insert_min(unsigned long long *keys, unsigned long long *vals,
-                                              uint64_t mask, uint64_t key, uint64_t val,
-                                              uint32_t *err) {
+// the same key (values must start at ~0).  Returns the slot, or -1 if the table
+// is full (probing is bounded by the capacity; *err is set).
+__device__ __forceinline__ int64_t ht_insert_min(unsigned long long *keys, unsigned long long *vals,
+                                                 uint64_t mask, uint64_t key, uint64_t val, uint32_t *err) {
     uint64_t s = mix64(key) & mask;
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         unsigned long long prev = atomicCAS(&keys[s], KEY_EMPTY, (unsigned long long)key);
         if (prev == KEY_EMPTY || prev == key) {
             atomicMin(&vals[s], (unsigned long long)val);
-            return;
+            return (int64_t)s;
         }
         s = (s + 1) & mask;
     }
     atomicOr(err, 1u);
+    return -1;
 }
 
 // Insert-if-absent for a table whose keys are unique (no value race).

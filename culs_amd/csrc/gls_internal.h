@@ -101,10 +101,10 @@ int aiger_write(const char *path, int32_t nPIs, const std::vector<uint32_t> &f0,
 // releases it) and fill `out` with fresh pool memory, out.level included.
 int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, gls_op_info *info);
 int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart);
-int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, int32_t *depth,
+int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
                 gls_op_info *info);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
-                       uint32_t *d_cuts, int32_t *d_nref);
+                       uint32_t *d_cuts, int32_t *d_nref, int mode);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,
                  int useZeros, int32_t *d_best);
 void dev_graph_free(gls_ctx *ctx, DevGraph &g);

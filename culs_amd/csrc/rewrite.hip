@@ -22,6 +22,7 @@
 #include "gls_internal.h"
 #include "dev_common.h"
 #include "rwlib_forest.h"
+#include <hip/hip_cooperative_groups.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -34,7 +35,6 @@ constexpr int NCUT = 8;
 constexpr int MFFC_TAB = 8;
 constexpr int NMATCH = 54;
 constexpr int EVAL_BLOCK = 64;
-constexpr int CUT_BLOCK = 128;
 constexpr int PLAN_MAXM = 16;
 
 struct Lib {
@@ -226,53 +226,174 @@ __global__ void k_cuts_pi(int32_t nPIs, const int32_t *__restrict__ nRef, uint32
     store_cuts(cuts + (size_t)v * NCUT * 5, c);
 }
 
-// One thread per node of one level; the node's 8 slots live in LDS.
-__global__ void __launch_bounds__(CUT_BLOCK) k_cuts_level(int32_t start, int32_t count,
-                                                          const uint32_t *__restrict__ f0,
-                                                          const uint32_t *__restrict__ f1,
-                                                          const int32_t *__restrict__ nRef, uint32_t *cuts) {
-    __shared__ DCut sh[CUT_BLOCK][NCUT];
-    int i = blockIdx.x * CUT_BLOCK + threadIdx.x;
-    if (i >= count) return;
-    const int idx = start + i;
-    DCut *cs = sh[threadIdx.x];
-    for (int s = 0; s < NCUT; s++) { cs[s].used = 0; cs[s].nLeaves = 0; cs[s].tt = 0; cs[s].value = 0; }
-    cs[0].used = 1; cs[0].tt = 0xAAAA; cs[0].nLeaves = 1; cs[0].leaves[0] = (uint32_t)idx;
-    cs[0].value = cut_value(cs[0], nRef);
-    const uint32_t l0 = f0[idx], l1 = f1[idx];
-    const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
-    const uint32_t *p0 = cuts + (size_t)in0 * NCUT * 5, *p1 = cuts + (size_t)in1 * NCUT * 5;
-    bool done = false;
-    for (int a_i = 0; a_i < NCUT && !done; a_i++) {
-        DCut a;
-        unpack_cut(p0 + 5 * a_i, a);
-        if (!a.used) continue;
-        const uint32_t sa = cut_sign(a);
-        for (int b_j = 0; b_j < NCUT; b_j++) {
-            DCut b;
-            unpack_cut(p1 + 5 * b_j, b);
-            if (!b.used) continue;
-            if (__popc(sa | cut_sign(b)) > 4) continue;
-            int id = find_cut(cs);
-            DCut m = cs[id];
-            if (a.nLeaves >= b.nLeaves) { if (!merge_ordered(a, b, m)) { cs[id] = m; continue; } }
-            else if (!merge_ordered(b, a, m)) { cs[id] = m; continue; }
-            m.used = 1;
-            cs[id] = m;
-            if (cut_filter(cs, id)) continue;
-            uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
-            t0 = tt_stretch(t0, a.nLeaves, truth_phase(a, cs[id]));
-            t1 = tt_stretch(t1, b.nLeaves, truth_phase(b, cs[id]));
-            cs[id].tt = (t0 & t1) & 0xFFFF;
-            DCut mm = cs[id];
-            int shrunk = minimize_support(mm);
-            cs[id] = mm;
-            if (shrunk) cut_filter(cs, id);
-            cs[id].value = cut_value(cs[id], nRef);
-            if (cs[id].nLeaves < 2) { done = true; break; }
+// One wavefront per node.  Lane l = 8*i + j owns the pair (cut i of fanin0,
+// cut j of fanin1): the 64 merges, truth tables, support reductions and cut
+// values are computed in parallel.  The reference inserts the candidates one
+// by one in (i, j) order with an order-dependent slot policy (FindCut evicts,
+// CutFilter drops dominated cuts, rewrite.cu:84-193, 268-294); that replay runs
+// wave-uniformly over the surviving candidates with the 8 slots held in lanes
+// 0..7, using ballots for the per-slot tests, so the result equals the
+// sequential one exactly.
+constexpr int CUTW_BLOCK = 256;
+enum : int { C_SKIP = 0, C_FAIL = 1, C_OK = 2 };
+
+__device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ __forceinline__ bool leaf_in(uint32_t x, uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, int n) {
+    return (n > 0 && x == s0) || (n > 1 && x == s1) || (n > 2 && x == s2) || (n > 3 && x == s3);
+}
+
+struct Slot {
+    uint32_t l0, l1, l2, l3, tt, value;
+    int n, used;
+};
+
+// CutFilter (rewrite.cu:178-193) of the cut in slot `id` (leaves N, nn) against
+// the other used slots.  Scanning slots in order, a larger slot that contains N
+// is dropped; the first slot of no more leaves that N contains drops N and ends
+// the scan.  Returns true if N was dropped.
+__device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t n0, uint32_t n1, uint32_t n2,
+                                            uint32_t n3, int nn) {
+    const bool mine = lane < NCUT && lane != id && s.used;
+    const bool bigger = s.n > nn;
+    bool d1 = false, d2 = false;
+    if (mine) {
+        if (bigger) {
+            d1 = (nn < 1 || leaf_in(n0, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 2 || leaf_in(n1, s.l0, s.l1, s.l2, s.l3, s.n)) &&
+                 (nn < 3 || leaf_in(n2, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 4 || leaf_in(n3, s.l0, s.l1, s.l2, s.l3, s.n));
+        } else {
+            d2 = (s.n < 1 || leaf_in(s.l0, n0, n1, n2, n3, nn)) && (s.n < 2 || leaf_in(s.l1, n0, n1, n2, n3, nn)) &&
+                 (s.n < 3 || leaf_in(s.l2, n0, n1, n2, n3, nn)) && (s.n < 4 || leaf_in(s.l3, n0, n1, n2, n3, nn));
         }
     }
-    store_cuts(cuts + (size_t)idx * NCUT * 5, cs);
+    const uint64_t m2 = __ballot(d2);
+    const int k = m2 ? __ffsll((unsigned long long)m2) - 1 : NCUT;
+    if (d1 && lane < k) s.used = 0;
+    if (k < NCUT) {
+        if (lane == id) s.used = 0;
+        return true;
+    }
+    return false;
+}
+
+// Cut set of node idx, computed by one whole wavefront (lane = thread lane).
+__device__ __forceinline__ void cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
+                                         const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
+                                         uint32_t *cuts) {
+    const uint32_t l0 = f0[idx], l1 = f1[idx];
+    const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
+    DCut a, b;
+    unpack_cut(cuts + ((size_t)in0 * NCUT + (lane >> 3)) * 5, a);
+    unpack_cut(cuts + ((size_t)in1 * NCUT + (lane & 7)) * 5, b);
+    // ---- phase 1: every pair in parallel
+    int status = C_SKIP, shrunk = 0;
+    DCut m, mm;
+    m.leaves[0] = m.leaves[1] = m.leaves[2] = m.leaves[3] = 0;
+    m.nLeaves = 0; m.tt = 0; m.value = 0; m.used = 0;
+    mm = m;
+    if (a.used && b.used && __popc(cut_sign(a) | cut_sign(b)) <= 4) {
+        int ok = a.nLeaves >= b.nLeaves ? merge_ordered(a, b, m) : merge_ordered(b, a, m);
+        if (!ok) {
+            status = C_FAIL;
+        } else {
+            status = C_OK;
+            uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
+            t0 = tt_stretch(t0, a.nLeaves, truth_phase(a, m));
+            t1 = tt_stretch(t1, b.nLeaves, truth_phase(b, m));
+            m.tt = (t0 & t1) & 0xFFFF;
+            mm = m;
+            shrunk = minimize_support(mm);
+            mm.value = cut_value(mm, nRef);
+        }
+    }
+    // ---- phase 2: replay the insertions in (i, j) order
+    Slot s;
+    s.l0 = lane == 0 ? (uint32_t)idx : 0u;
+    s.l1 = s.l2 = s.l3 = 0;
+    s.n = lane == 0 ? 1 : 0;
+    s.tt = lane == 0 ? 0xAAAAu : 0u;
+    s.value = lane == 0 ? 1001u : 0u;
+    s.used = lane == 0 ? 1 : 0;
+    uint64_t todo = __ballot(status != C_SKIP);
+    while (todo) {
+        const int l = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        const int st = rl(status, l);
+        // FindCut (rewrite.cu:84-109)
+        const uint64_t usedMask = __ballot(lane < NCUT && s.used) & 0xFFull;
+        int id;
+        if (usedMask != 0xFFull) {
+            id = __ffsll((unsigned long long)(~usedMask & 0xFFull)) - 1;
+        } else {
+            id = -1;
+            for (int cat = 0; cat < 3 && id < 0; cat++) {
+                const bool in = lane < NCUT && (cat == 0 ? s.n > 2 : cat == 1 ? s.n == 2 : s.n < 2);
+                uint32_t key = in ? ((s.value << 3) | (uint32_t)lane) : 0xFFFFFFFFu;
+                for (int off = 1; off < NCUT; off <<= 1) {
+                    uint32_t o = (uint32_t)__shfl_xor((int)key, off);
+                    key = o < key ? o : key;
+                }
+                key = (uint32_t)rl((int)key, 0);
+                if (key != 0xFFFFFFFFu) id = (int)(key & 7u);
+            }
+        }
+        if (lane == id) s.used = 0;
+        if (st == C_FAIL) continue;
+        const int nn = rl(m.nLeaves, l);
+        const uint32_t n0 = (uint32_t)rl((int)m.leaves[0], l), n1 = (uint32_t)rl((int)m.leaves[1], l);
+        const uint32_t n2 = (uint32_t)rl((int)m.leaves[2], l), n3 = (uint32_t)rl((int)m.leaves[3], l);
+        if (lane == id) { s.l0 = n0; s.l1 = n1; s.l2 = n2; s.l3 = n3; s.n = nn; s.used = 1; }
+        if (filter_wave(s, lane, id, n0, n1, n2, n3, nn)) continue;
+        const uint32_t ttv = (uint32_t)rl((int)m.tt, l);
+        if (lane == id) s.tt = ttv;
+        int fn = nn;
+        if (rl(shrunk, l)) {
+            fn = rl(mm.nLeaves, l);
+            const uint32_t q0 = (uint32_t)rl((int)mm.leaves[0], l), q1 = (uint32_t)rl((int)mm.leaves[1], l);
+            const uint32_t q2 = (uint32_t)rl((int)mm.leaves[2], l), q3 = (uint32_t)rl((int)mm.leaves[3], l);
+            const uint32_t qt = (uint32_t)rl((int)mm.tt, l);
+            if (lane == id) { s.l0 = q0; s.l1 = q1; s.l2 = q2; s.l3 = q3; s.n = fn; s.tt = qt; }
+            filter_wave(s, lane, id, q0, q1, q2, q3, fn);
+        }
+        const uint32_t val = (uint32_t)rl((int)mm.value, l);
+        if (lane == id) s.value = val;
+        if (fn < 2) break;
+    }
+    // ---- phase 3: lanes 0..7 store the slots
+    if (lane < NCUT) {
+        uint32_t *o = cuts + ((size_t)idx * NCUT + lane) * 5;
+        const bool u = s.used;
+        o[0] = (u && s.n > 0) ? s.l0 : 0u;
+        o[1] = (u && s.n > 1) ? s.l1 : 0u;
+        o[2] = (u && s.n > 2) ? s.l2 : 0u;
+        o[3] = (u && s.n > 3) ? s.l3 : 0u;
+        o[4] = u ? ((s.tt & 0xFFFF) | (s.value & 0x7FF) << 16 | (uint32_t)(s.n & 7) << 27 | 1u << 30) : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_t count,
+                                                           const uint32_t *__restrict__ f0,
+                                                           const uint32_t *__restrict__ f1,
+                                                           const int32_t *__restrict__ nRef, uint32_t *cuts) {
+    const int k = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= count) return;    // uniform per wavefront
+    cut_node(start + k, threadIdx.x & 63, f0, f1, nRef, cuts);
+}
+
+// All levels in one cooperative launch: the wavefronts of the grid stride over
+// each level's nodes, and a grid barrier separates the levels.
+__global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t l1, const int32_t *__restrict__ starts,
+                                                            const uint32_t *__restrict__ f0,
+                                                            const uint32_t *__restrict__ f1,
+                                                            const int32_t *__restrict__ nRef, uint32_t *cuts) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int nw = gridDim.x * (CUTW_BLOCK / 64);
+    const int gw = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
+    for (int l = l0; l < l1; l++) {
+        const int s = starts[l], e = starts[l + 1];
+        for (int v = s + gw; v < e; v += nw) cut_node(v, threadIdx.x & 63, f0, f1, nRef, cuts);
+        grid.sync();
+    }
 }
 
 __global__ void k_nref(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
@@ -845,7 +966,7 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
 }
 
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
-                       int32_t *d_nref) {
+                       int32_t *d_nref, int mode) {
     hipStream_t st = ctx->stream;
     const int32_t n = g.nObjs;
     GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
@@ -855,10 +976,38 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     // levelStart[l] = first id of level l (ids 1.. are level-major); the last
     // entry is one past the last id.
     const int L = (int)levelStart.size() - 1;
-    for (int l = 1; l < L; l++) {
-        int32_t s = levelStart[l], e = levelStart[l + 1];
-        if (e <= s) continue;
-        k_cuts_level<<<blocks(e - s, CUT_BLOCK), CUT_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+    if (L <= 1) return GLS_OK;
+    bool coop = mode != GLS_RW_PERLEVEL_CUTS && L > 16;
+    int G = 0;
+    if (coop) {
+        int perCU = 0, dev = 0, nCU = 0, coopOk = 0;
+        GLS_HIP(ctx, hipGetDevice(&dev));
+        GLS_HIP(ctx, hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
+        GLS_HIP(ctx, hipDeviceGetAttribute(&coopOk, hipDeviceAttributeCooperativeLaunch, dev));
+        GLS_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cuts_levels, CUTW_BLOCK, 0));
+        G = std::min(perCU * nCU, 64);
+        coop = coopOk && G > 0;
+    }
+    if (coop) {
+        int32_t *d_starts = pool_get<int32_t>(ctx, levelStart.size());
+        if (!d_starts) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: level starts");
+        GLS_HIP(ctx, hipMemcpyAsync(d_starts, levelStart.data(), sizeof(int32_t) * levelStart.size(),
+                                    hipMemcpyHostToDevice, st));
+        int32_t l0 = 1, l1 = L;
+        const uint32_t *a0 = g.f0, *a1 = g.f1;
+        const int32_t *ar = d_nref;
+        uint32_t *ac = d_cuts;
+        void *args[] = {&l0, &l1, &d_starts, &a0, &a1, &ar, &ac};
+        hipError_t e = hipLaunchCooperativeKernel((const void *)k_cuts_levels, dim3(G), dim3(CUTW_BLOCK), args, 0, st);
+        // the pool block is reused only by later work on the same stream
+        pool_put(ctx, d_starts);
+        if (e != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cooperative cut launch: %s", hipGetErrorString(e));
+    } else {
+        for (int l = 1; l < L; l++) {
+            int32_t s = levelStart[l], e = levelStart[l + 1];
+            if (e <= s) continue;
+            k_cuts_level<<<blocks(e - s, CUTW_BLOCK / 64), CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+        }
     }
     GLS_HIP(ctx, hipGetLastError());
     return GLS_OK;
@@ -904,7 +1053,8 @@ int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const 
     return GLS_OK;
 }
 
-int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, int32_t *depth, gls_op_info *info) {
+int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
+                gls_op_info *info) {
     GLS_TRY(ctx_ensure_library(ctx));
     hipStream_t st = ctx->stream;
     Timer tm{ctx};
@@ -926,7 +1076,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, DevGraph &out, i
         unsigned long long *d_gain = t.get<unsigned long long>(1);
         if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n); }
         GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 16, st));
-        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref));
+        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & GLS_RW_PERLEVEL_CUTS));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};
         GLS_TRY(build_table(ctx, t, g, gv, d_err));

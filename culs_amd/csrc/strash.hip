@@ -37,12 +37,12 @@ __global__ void k_count_refs(int32_t n, int32_t nPIs, const uint32_t *__restrict
     for (int64_t i = tid; i < nPOs; i += stride) atomicAdd(&ref[LID(pos[i])], 1u);
 }
 
-__global__ void k_seed_zero(int32_t n, int32_t nPIs, const uint32_t *__restrict__ cnt, SweepCtl *ctl,
+__global__ void k_seed_zero(int32_t n, int32_t nPIs, const uint32_t *__restrict__ cnt, uint32_t *counter,
                             uint32_t *order, const uint8_t *__restrict__ dead) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
         if (dead && dead[v]) continue;
-        if (cnt[v] == 0) order[atomicAdd(&ctl->curSize, 1u)] = (uint32_t)v;
+        if (cnt[v] == 0) order[atomicAdd(counter, 1u)] = (uint32_t)v;
     }
 }
 
@@ -90,88 +90,218 @@ __global__ void k_kahn_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict_
     }
 }
 
-struct StrashArgs {
+struct WaveArgs {
     const uint32_t *f0, *f1;
     int32_t nPIs;
-    uint32_t *res;                  // literal over representative ids, INVALID until resolved
-    unsigned long long *keyOf;      // key of non-trivial nodes
-    uint32_t *level, *wave;
-    unsigned long long *htK, *htV;
+    unsigned long long *rl;         // per node: literal over representative ids | level << 32;
+                                    // the literal is INVALID32 until the representative is known
+    uint32_t *slotOf, *wave;        // table slot of non-trivial nodes, wave of every live node
+    unsigned long long *ht;         // slot s: ht[2s] = key, ht[2s + 1] = min (wave << 32 | id)
     uint64_t mask;
     const uint32_t *adjOff, *adj;
     uint32_t *pending, *order;
+    uint32_t *fcnt, *fstart;        // size and start (in order[]) of every wave's frontier
+    uint32_t *depth, *err;
 };
 
-__device__ __forceinline__ uint32_t map_lit(const StrashArgs &a, uint32_t lit) {
-    uint32_t u = LID(lit);
-    if (u <= (uint32_t)a.nPIs) return lit;
-    uint32_t r = a.res[u];
-    if (r == INVALID32) {
-        // u belongs to the previous wave: its representative is final in the table
-        unsigned long long val = 0;
-        ht_find(a.htK, a.htV, a.mask, a.keyOf[u], &val);
-        r = 2u * (uint32_t)(val & 0xFFFFFFFFull);
-    }
+__device__ __forceinline__ unsigned long long pack_rl(uint32_t lit, uint32_t lev) {
+    return (static_cast<unsigned long long>(lev) << 32) | lit;
+}
+
+template <int SCOPE>
+__device__ __forceinline__ uint32_t rep_of_slot(const WaveArgs &a, uint32_t slot) {
+    unsigned long long val = __hip_atomic_load(&a.ht[2 * (size_t)slot + 1], __ATOMIC_RELAXED, SCOPE);
+    return 2u * (uint32_t)(val & 0xFFFFFFFFull);
+}
+
+// Fanin literal over representatives, and its level.  A fanin of the previous
+// wave may still be unresolved; its table value is final by now.
+template <int SCOPE>
+__device__ __forceinline__ uint32_t map_fanin(const WaveArgs &a, uint32_t lit, uint32_t &lev) {
+    const uint32_t u = LID(lit);
+    if (u <= (uint32_t)a.nPIs) { lev = 0; return lit; }
+    const unsigned long long x = a.rl[u];
+    uint32_t r = (uint32_t)x;
+    lev = (uint32_t)(x >> 32);
+    if (r == INVALID32) r = rep_of_slot<SCOPE>(a, a.slotOf[u]);
     return r ^ LC(lit);
 }
 
-// One wave: resolve the previous wave's representatives, hash the current
-// wave, and release the fanouts whose last fanin was in this wave.
-__global__ void k_strash_round(uint32_t k, SweepCtl *ctl, StrashArgs a) {
-    uint32_t ps = ctl->prevStart, pn = ctl->prevSize, cs = ctl->curStart, cn = ctl->curSize;
-    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (uint32_t i = tid; i < pn; i += stride) {
-        uint32_t v = a.order[ps + i];
-        if (a.res[v] == INVALID32) {
-            unsigned long long val = 0;
-            ht_find(a.htK, a.htV, a.mask, a.keyOf[v], &val);
-            a.res[v] = 2u * (uint32_t)(val & 0xFFFFFFFFull);
-        }
+// Hashes node v of wave k.  Trivial ANDs (hash_table.h:435-449 rules) are
+// resolved at once; the others are inserted with value (k << 32 | v) kept by
+// minimum, so the representative of duplicates is the smallest (wave, id).
+// Returns true for a non-trivial node (slot and level filled, rl not yet).
+template <int SCOPE>
+__device__ __forceinline__ bool hash_node(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t &slotOut,
+                                          uint32_t &levOut) {
+    uint32_t la, lb;
+    uint32_t l0 = map_fanin<SCOPE>(a, a.f0[v], la), l1 = map_fanin<SCOPE>(a, a.f1[v], lb);
+    if (l0 > l1) { uint32_t t = l0; l0 = l1; l1 = t; t = la; la = lb; lb = t; }
+    a.wave[v] = k;
+    uint32_t r = INVALID32, rlev = 0;
+    if (l0 == l1) { r = l0; rlev = la; }
+    else if (l0 == (l1 ^ 1u)) r = 0;
+    else if (l0 == 1u) { r = l1; rlev = lb; }
+    else if (l0 == 0u) r = 0;
+    if (r != INVALID32) {
+        a.rl[v] = pack_rl(r, rlev);
+        return false;
     }
-    for (uint32_t i = tid; i < cn; i += stride) {
-        uint32_t v = a.order[cs + i];
-        uint32_t l0 = map_lit(a, a.f0[v]), l1 = map_lit(a, a.f1[v]);
-        if (l0 > l1) { uint32_t t = l0; l0 = l1; l1 = t; }
-        a.wave[v] = k;
-        uint32_t r = INVALID32;
-        if (l0 == l1) r = l0;
-        else if (l0 == (l1 ^ 1u)) r = 0;
-        else if (l0 == 1u) r = l1;
-        else if (l0 == 0u) r = 0;
-        if (r != INVALID32) {
-            a.res[v] = r;
-            uint32_t u = LID(r);
-            a.level[v] = u <= (uint32_t)a.nPIs ? 0u : a.level[u];
-        } else {
-            uint64_t key = and_key(l0, l1);
-            a.keyOf[v] = key;
-            uint32_t la = LID(l0) <= (uint32_t)a.nPIs ? 0u : a.level[LID(l0)];
-            uint32_t lb = LID(l1) <= (uint32_t)a.nPIs ? 0u : a.level[LID(l1)];
-            a.level[v] = 1u + (la > lb ? la : lb);
-            ht_insert_min(a.htK, a.htV, a.mask, key, (static_cast<uint64_t>(k) << 32) | v, &ctl->err);
+    levOut = 1u + (la > lb ? la : lb);
+    const uint64_t key = and_key(l0, l1), val = (static_cast<uint64_t>(k) << 32) | v;
+    uint64_t s = mix64(key) & a.mask;
+    int64_t slot = -1;
+    for (uint64_t probe = 0; probe <= a.mask; ++probe) {
+        unsigned long long prev = KEY_EMPTY;
+        __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, SCOPE);
+        if (prev == KEY_EMPTY || prev == key) {
+            __hip_atomic_fetch_min(&a.ht[2 * s + 1], (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
+            slot = (int64_t)s;
+            break;
         }
-        for (uint32_t e = a.adjOff[v], ee = a.adjOff[v + 1]; e < ee; ++e) {
-            uint32_t w = a.adj[e];
-            if (atomicSub(&a.pending[w], 1u) == 1u) a.order[cs + cn + atomicAdd(&ctl->nextSize, 1u)] = w;
-        }
+        s = (s + 1) & a.mask;
     }
-    sweep_round_end(ctl, k);
+    if (slot < 0) atomicOr(a.err, 1u);
+    slotOut = slot < 0 ? 0u : (uint32_t)slot;
+    a.slotOf[v] = slotOut;
+    return true;
 }
 
-__global__ void k_resolve_all(int32_t n, int32_t nPIs, const uint8_t *__restrict__ dead, uint32_t *res,
-                              const unsigned long long *__restrict__ keyOf,
-                              const unsigned long long *__restrict__ htK,
-                              const unsigned long long *__restrict__ htV, uint64_t mask, uint8_t *surv) {
+// Releases the fanouts of v whose last pending fanin is v.  Up to four
+// decrements are in flight before any result is inspected.
+template <int SCOPE, class Emit>
+__device__ __forceinline__ void release(const WaveArgs &a, uint32_t v, Emit emit) {
+    const uint32_t e0 = a.adjOff[v], ee = a.adjOff[v + 1];
+    for (uint32_t e = e0; e < ee; e += 4) {
+        uint32_t w[4], r[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) w[j] = (e + j < ee) ? a.adj[e + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            r[j] = (e + j < ee) ? __hip_atomic_fetch_add(&a.pending[w[j]], 0xFFFFFFFFu, __ATOMIC_RELAXED, SCOPE) : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (e + j < ee && r[j] == 1u) emit(w[j]);
+    }
+}
+
+template <int SCOPE>
+__device__ __forceinline__ void resolve(const WaveArgs &a, uint32_t v) {
+    const unsigned long long x = a.rl[v];
+    if ((uint32_t)x == INVALID32) a.rl[v] = pack_rl(rep_of_slot<SCOPE>(a, a.slotOf[v]), (uint32_t)(x >> 32));
+}
+
+// One wave across the whole device (wide frontiers): items below pn resolve
+// the previous wave, the rest hash this wave.
+__global__ void k_wave_multi(uint32_t k, WaveArgs a) {
+    const uint32_t pn = a.fcnt[k - 1], ps = a.fstart[k - 1], cn = a.fcnt[k], cs = a.fstart[k];
+    const uint32_t nb = cs + cn;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (tid == 0) {
+        a.fstart[k + 1] = nb;
+        if (cn > 0) atomicMax(a.depth, k);
+    }
+    for (uint32_t i = tid; i < pn + cn; i += stride) {
+        if (i < pn) {
+            resolve<__HIP_MEMORY_SCOPE_AGENT>(a, a.order[ps + i]);
+            continue;
+        }
+        const uint32_t v = a.order[cs + i - pn];
+        uint32_t slot = 0, lev = 0;
+        if (hash_node<__HIP_MEMORY_SCOPE_AGENT>(a, k, v, slot, lev)) a.rl[v] = pack_rl(INVALID32, lev);
+        release<__HIP_MEMORY_SCOPE_AGENT>(a, v, [&](uint32_t w) { a.order[nb + atomicAdd(&a.fcnt[k + 1], 1u)] = w; });
+    }
+}
+
+// Consecutive narrow waves inside one workgroup.  Waves are separated by
+// workgroup barriers instead of kernel boundaries, the frontier lives in LDS,
+// atomics are workgroup-scoped (performed in the local L2), and each wave's
+// representatives are resolved right after its inserts, so the next wave maps
+// a fanin with a single load.  Runs while the frontier is non-empty and at
+// most `thresh` (<= FRONT_MAX) wide; *kOut = next wave to run.
+constexpr int WG1 = 1024;
+constexpr int FRONT_MAX = 4096;
+constexpr int WG1_ITEMS = FRONT_MAX / WG1;
+__global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh, WaveArgs a, uint32_t *kOut) {
+    __shared__ uint32_t s_front[2][FRONT_MAX];
+    __shared__ uint32_t s_next;
+    const uint32_t tid = threadIdx.x;
+    uint32_t k = k0;
+    const uint32_t pn = a.fcnt[k - 1], ps = a.fstart[k - 1];
+    uint32_t cn = a.fcnt[k], cs = a.fstart[k];
+    // the previous wave may come from the grid-wide kernel, unresolved
+    for (uint32_t i = tid; i < pn; i += WG1) resolve<__HIP_MEMORY_SCOPE_WORKGROUP>(a, a.order[ps + i]);
+    if (cn <= thresh)
+        for (uint32_t i = tid; i < cn; i += WG1) s_front[0][i] = a.order[cs + i];
+    __syncthreads();
+    int buf = 0;
+    uint32_t last = 0;
+    while (cn > 0 && cn <= thresh) {
+        const uint32_t nb = cs + cn;
+        if (tid == 0) s_next = 0;
+        __syncthreads();
+        // tasks [0, cn) hash node i, tasks [cn, 2cn) release the fanouts of
+        // node i - cn: both run at once on different threads
+        uint32_t myV[WG1_ITEMS], mySlot[WG1_ITEMS], myLev[WG1_ITEMS];
+        bool myNT[WG1_ITEMS];
+#pragma unroll
+        for (int j = 0; j < WG1_ITEMS; j++) {
+            const uint32_t i = tid + (uint32_t)j * WG1;
+            myNT[j] = false;
+            myV[j] = 0;
+            if (i < cn) {
+                const uint32_t v = s_front[buf][i];
+                myV[j] = v;
+                myNT[j] = hash_node<__HIP_MEMORY_SCOPE_WORKGROUP>(a, k, v, mySlot[j], myLev[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * WG1_ITEMS; j++) {
+            const uint32_t i = tid + (uint32_t)j * WG1;
+            if (i >= cn && i < 2 * cn) {
+                const uint32_t v = s_front[buf][i - cn];
+                release<__HIP_MEMORY_SCOPE_WORKGROUP>(a, v, [&](uint32_t w) {
+                    const uint32_t p = atomicAdd(&s_next, 1u);
+                    if (p < FRONT_MAX) s_front[buf ^ 1][p] = w;
+                    a.order[nb + p] = w;
+                });
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < WG1_ITEMS; j++)
+            if (myNT[j]) a.rl[myV[j]] = pack_rl(rep_of_slot<__HIP_MEMORY_SCOPE_WORKGROUP>(a, mySlot[j]), myLev[j]);
+        const uint32_t nn = s_next;
+        if (tid == 0) {
+            a.fstart[k + 1] = nb;
+            a.fcnt[k + 1] = nn;
+        }
+        last = k;
+        cs = nb;
+        cn = nn;
+        ++k;
+        buf ^= 1;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *kOut = k;
+        if (last) atomicMax(a.depth, last);
+    }
+}
+
+__global__ void k_resolve_all(int32_t n, int32_t nPIs, const uint8_t *__restrict__ dead, unsigned long long *rl,
+                              const uint32_t *__restrict__ slotOf, const unsigned long long *__restrict__ ht,
+                              uint8_t *surv) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = tid; v < n; v += stride) {
         uint8_t s = 0;
         if (v > nPIs && !dead[v]) {
-            uint32_t r = res[v];
+            unsigned long long x = rl[v];
+            uint32_t r = (uint32_t)x;
             if (r == INVALID32) {
-                unsigned long long val = 0;
-                ht_find(htK, htV, mask, keyOf[v], &val);
-                r = 2u * (uint32_t)(val & 0xFFFFFFFFull);
-                res[v] = r;
+                r = 2u * (uint32_t)(ht[2 * (size_t)slotOf[v] + 1] & 0xFFFFFFFFull);
+                rl[v] = (x & 0xFFFFFFFF00000000ull) | r;
             }
             s = (r == 2u * (uint32_t)v);
         }
@@ -192,34 +322,35 @@ __global__ void k_new_ids(int32_t n, int32_t nPIs, int32_t m, const uint32_t *__
     for (int64_t p = tid; p < m; p += stride) newId[sorted[p]] = (uint32_t)(nPIs + 1 + p);
 }
 
-__device__ __forceinline__ uint32_t final_lit(uint32_t lit, int32_t nPIs, const uint32_t *res,
+__device__ __forceinline__ uint32_t final_lit(uint32_t lit, int32_t nPIs, const unsigned long long *rl,
                                               const uint32_t *newId) {
     uint32_t u = LID(lit);
-    uint32_t r = (u <= (uint32_t)nPIs) ? 2u * u : res[u];
+    uint32_t r = (u <= (uint32_t)nPIs) ? 2u * u : (uint32_t)rl[u];
     r ^= LC(lit);
     return 2u * newId[LID(r)] | LC(r);
 }
 
 __global__ void k_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ sorted,
                        const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
-                       const uint32_t *__restrict__ res, const uint32_t *__restrict__ newId,
-                       const uint32_t *__restrict__ level, uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
+                       const unsigned long long *__restrict__ rl, const uint32_t *__restrict__ newId,
+                       uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = tid; p < m; p += stride) {
         uint32_t v = sorted[p];
-        uint32_t a = final_lit(f0[v], nPIs, res, newId), b = final_lit(f1[v], nPIs, res, newId);
+        uint32_t a = final_lit(f0[v], nPIs, rl, newId), b = final_lit(f1[v], nPIs, rl, newId);
         if (a > b) { uint32_t t = a; a = b; b = t; }
         int64_t o = nPIs + 1 + p;
         of0[o] = a;
         of1[o] = b;
-        olevel[o] = level[v];
+        olevel[o] = (uint32_t)(rl[v] >> 32);
     }
 }
 
 __global__ void k_map_pos(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict__ pos,
-                          const uint32_t *__restrict__ res, const uint32_t *__restrict__ newId, uint32_t *opos) {
+                          const unsigned long long *__restrict__ rl, const uint32_t *__restrict__ newId,
+                          uint32_t *opos) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = tid; i < nPOs; i += stride) opos[i] = final_lit(pos[i], nPIs, res, newId);
+    for (int64_t i = tid; i < nPOs; i += stride) opos[i] = final_lit(pos[i], nPIs, rl, newId);
 }
 
 struct Tmp {
@@ -273,29 +404,29 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     Tmp t{ctx};
     uint32_t *ref = t.get<uint32_t>(n), *order = t.get<uint32_t>(n), *pending = t.get<uint32_t>(n);
     uint32_t *cnt = t.get<uint32_t>(n + 1), *adjOff = t.get<uint32_t>(n + 1), *cursor = t.get<uint32_t>(n + 1);
-    uint32_t *adj = t.get<uint32_t>(2 * (size_t)n + 1), *res = t.get<uint32_t>(n);
-    uint32_t *level = t.get<uint32_t>(n), *wave = t.get<uint32_t>(n), *newId = t.get<uint32_t>(n);
+    uint32_t *adj = t.get<uint32_t>(2 * (size_t)n + 1);
+    unsigned long long *rl = t.get<unsigned long long>(n);
+    uint32_t *wave = t.get<uint32_t>(n), *newId = t.get<uint32_t>(n);
     uint32_t *survIds = t.get<uint32_t>(n), *survWave = t.get<uint32_t>(n);
     uint32_t *sortIds = t.get<uint32_t>(n), *sortWave = t.get<uint32_t>(n), *d_count = t.get<uint32_t>(4);
     uint8_t *dead = t.get<uint8_t>(n), *surv = t.get<uint8_t>(n);
-    unsigned long long *keyOf = t.get<unsigned long long>(n);
+    uint32_t *slotOf = t.get<uint32_t>(n), *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
+    uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
     SweepCtl *ctl = t.get<SweepCtl>(1);
     uint64_t cap = 1024;
     while (cap < 2ull * (uint64_t)n) cap <<= 1;
-    unsigned long long *htK = t.get<unsigned long long>(cap), *htV = t.get<unsigned long long>(cap);
-    if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !res || !level || !wave ||
-        !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !keyOf ||
-        !ctl || !htK || !htV)
+    unsigned long long *ht = t.get<unsigned long long>(2 * cap);
+    if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
+        !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !slotOf ||
+        !fcnt || !fstart || !d_misc || !ctl || !ht)
         return gls_fail(ctx, GLS_ERR_NOMEM, "strash: device allocation failed (n=%d)", n);
 
     GLS_HIP(ctx, hipMemsetAsync(ref, 0, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(dead, 0, n, st));
     GLS_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), st));
-    GLS_HIP(ctx, hipMemsetAsync(res, 0xFF, sizeof(uint32_t) * n, st));
-    GLS_HIP(ctx, hipMemsetAsync(level, 0, sizeof(uint32_t) * n, st));
+    GLS_HIP(ctx, hipMemsetAsync(rl, 0xFF, sizeof(unsigned long long) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(wave, 0, sizeof(uint32_t) * n, st));
-    GLS_HIP(ctx, hipMemsetAsync(htK, 0xFF, sizeof(unsigned long long) * cap, st));
-    GLS_HIP(ctx, hipMemsetAsync(htV, 0xFF, sizeof(unsigned long long) * cap, st));
+    GLS_HIP(ctx, hipMemsetAsync(ht, 0xFF, sizeof(unsigned long long) * 2 * cap, st));
     GLS_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(SweepCtl), st));
     const int gN = grid_for(n);
     SweepCtl h_ctl{};
@@ -303,7 +434,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
 
     // 1. fanout counts and dangling cascade
     k_count_refs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, nPOs, in.pos, ref);
-    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, ref, ctl, order, nullptr);
+    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, ref, &ctl->curSize, order, nullptr);
     GLS_HIP(ctx, hipGetLastError());
     GLS_HIP(ctx, hipMemcpyAsync(&h_ctl, ctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, st));
     GLS_HIP(ctx, hipStreamSynchronize(st));
@@ -314,8 +445,10 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     }
 
     const uint32_t nDead = h_ctl.processed;
-    // 2. fanout CSR over live AND nodes and the first frontier
-    GLS_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(SweepCtl), st));
+    // 2. fanout CSR over live AND nodes and the first frontier (wave 1)
+    GLS_HIP(ctx, hipMemsetAsync(fcnt, 0, sizeof(uint32_t) * (n + 4), st));
+    GLS_HIP(ctx, hipMemsetAsync(fstart, 0, sizeof(uint32_t) * (n + 4), st));
+    GLS_HIP(ctx, hipMemsetAsync(d_misc, 0, sizeof(uint32_t) * 8, st));
     k_kahn_count<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cnt, pending);
     size_t tb = 0;
     GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
@@ -324,27 +457,44 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
     k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj);
-    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, ctl, order, dead);
+    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
     GLS_HIP(ctx, hipGetLastError());
 
-    // 3. wave sweep
-    StrashArgs a{in.f0, in.f1, nPIs, res, keyOf, level, wave, htK, htV, cap - 1, adjOff, adj, pending, order};
-    GLS_HIP(ctx, hipMemcpyAsync(&h_ctl, ctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, st));
-    GLS_HIP(ctx, hipStreamSynchronize(st));
-    uint32_t waves = 0;
-    if (h_ctl.curSize > 0) {
-        GLS_TRY(run_sweep(ctx, ctl, 1, [&](uint32_t k, uint32_t hint) {
-            k_strash_round<<<sweep_grid(hint), kBlock, 0, st>>>(k, ctl, a);
-        }, &h_ctl, &waves));
+    // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
+    //    ones as one grid-wide kernel each
+    WaveArgs a{in.f0, in.f1, nPIs, rl, slotOf, wave, ht, cap - 1, adjOff, adj, pending, order,
+               fcnt, fstart, &d_misc[0], &d_misc[1]};
+    const uint32_t kThresh = FRONT_MAX;
+    uint32_t k = 1, waves = 0, cur = 0;
+    for (;;) {
+        GLS_TRY(stream_copy(ctx, &cur, &fcnt[k], sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (cur == 0) break;
+        if (cur <= kThresh) {
+            k_waves_1wg<<<1, WG1, 0, st>>>(k, kThresh, a, &d_misc[2]);
+            GLS_HIP(ctx, hipGetLastError());
+            uint32_t kn = k;
+            GLS_TRY(stream_copy(ctx, &kn, &d_misc[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
+            waves += kn - k;
+            k = kn;
+        } else {
+            for (int b = 0; b < 4; b++, k++, waves++)
+                k_wave_multi<<<sweep_grid(cur), kBlock, 0, st>>>(k, a);
+            GLS_HIP(ctx, hipGetLastError());
+        }
     }
-    int32_t depth = (int32_t)h_ctl.depth;
+    uint32_t misc[2] = {0, 0};
+    GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+    if (misc[1]) return gls_fail(ctx, GLS_ERR_INTERNAL, "strash table overflow");
+    const int32_t depth = (int32_t)misc[0];
+    uint32_t processed = 0;
+    GLS_TRY(stream_copy(ctx, &processed, &fstart[depth + 1], sizeof(uint32_t), hipMemcpyDeviceToHost));
     const uint32_t nAnds = n > nPIs + 1 ? (uint32_t)(n - nPIs - 1) : 0u;
-    if (h_ctl.processed + nDead != nAnds)
+    if (processed + nDead != nAnds)
         return gls_fail(ctx, GLS_ERR_ARG, "strash: %u of %u AND nodes unreachable in level order (cycle)",
-                        nAnds - nDead - h_ctl.processed, nAnds - nDead);
+                        nAnds - nDead - processed, nAnds - nDead);
 
     // 4. survivors in (wave, id) order
-    k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, res, keyOf, htK, htV, cap - 1, surv);
+    k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, slotOf, ht, surv);
     GLS_HIP(ctx, hipGetLastError());
     size_t tb2 = 0;
     rocprim::counting_iterator<uint32_t> ids(0u);
@@ -373,9 +523,9 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     }
     k_new_ids<<<grid_for(m + nPIs + 1), kBlock, 0, st>>>(n, nPIs, (int32_t)m, sorted, newId);
     if (m > 0)
-        k_emit<<<grid_for(m), kBlock, 0, st>>>((int32_t)m, nPIs, sorted, in.f0, in.f1, res, newId, level,
+        k_emit<<<grid_for(m), kBlock, 0, st>>>((int32_t)m, nPIs, sorted, in.f0, in.f1, rl, newId,
                                                 out.f0, out.f1, out.level);
-    if (nPOs > 0) k_map_pos<<<grid_for(nPOs), kBlock, 0, st>>>(nPOs, nPIs, in.pos, res, newId, out.pos);
+    if (nPOs > 0) k_map_pos<<<grid_for(nPOs), kBlock, 0, st>>>(nPOs, nPIs, in.pos, rl, newId, out.pos);
     GLS_HIP(ctx, hipGetLastError());
     GLS_HIP(ctx, hipStreamSynchronize(st));
     if (depthOut) *depthOut = depth;

@@ -92,6 +92,18 @@ def test_rewrite_parity(gpu_ctx, name, spec):
         s = got
 
 
+@pytest.mark.parametrize("name,spec", CASES[1:], ids=[c[0] for c in CASES[1:]])
+def test_cut_launch_modes_agree(gpu_ctx, name, spec):
+    """One cooperative launch over all levels == one launch per level."""
+    log("cut launch modes: %s" % name)
+    a = make(gpu_ctx, spec)
+    a.strash()
+    b = a.copy()
+    a.rewrite()
+    b.rewrite(per_level_cuts=True)
+    assert same(a.arrays(), b.arrays())
+
+
 def test_rewrite_deterministic(gpu_ctx):
     log("rw determinism")
     outs = []
