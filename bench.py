@@ -48,6 +48,8 @@ def main():
                     help="ANDs of the CPU oracle sample (same generator)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cut-mode", type=int, default=0,
+                    help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the path is a single-device kernel pipeline "
@@ -72,7 +74,7 @@ def main():
 
     def step():
         c = src.copy()
-        c.rewrite()
+        c.rewrite(cut_mode=a.cut_mode)
         return c
 
     for _ in range(a.warmup):

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libgls.so")
 
 GLS_LIT_AIGER = 0
 GLS_LIT_CULS = 1
-GLS_RW_PERLEVEL_CUTS = 1
+GLS_RW_COOP_CUTS = 2
+GLS_RW_DATAFLOW_CUTS = 4
 
 
 class GlsError(RuntimeError):
@@ -206,11 +207,11 @@ class Aig:
         self.ctx.check(lib().gls_strash(self._h, 0), "gls_strash")
         return self
 
-    def rewrite(self, use_zeros=False, per_level_cuts=False):
-        """One rw pass (rw -z / rwz with use_zeros).  per_level_cuts selects one
-        kernel launch per level for cut enumeration instead of one cooperative
-        launch; the results are identical."""
-        flags = GLS_RW_PERLEVEL_CUTS if per_level_cuts else 0
+    def rewrite(self, use_zeros=False, cut_mode=0):
+        """One rw pass (rw -z / rwz with use_zeros).  cut_mode schedules cut
+        enumeration: 0 one launch per level, GLS_RW_COOP_CUTS one cooperative
+        launch, GLS_RW_DATAFLOW_CUTS fanin-driven; the results are identical."""
+        flags = cut_mode
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
         return self
 

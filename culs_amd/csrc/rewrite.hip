@@ -276,15 +276,68 @@ __device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t 
     return false;
 }
 
+// Fence-free record format of the dataflow schedule: every 64-bit word of a
+// slot carries a valid bit, so a reader polls the words themselves with
+// relaxed device-scope atomics and needs no ordering between words.
+//   w0 = leaf0 | leaf1 << 31 | V,  w1 = leaf2 | leaf3 << 31 | V,  w2 = meta | V
+constexpr unsigned long long TAG_V = 1ull << 63;
+constexpr uint32_t DF_SPIN_LIMIT = 1u << 22;
+
+__device__ __forceinline__ bool load_tagged(const unsigned long long *rec, DCut &c) {
+    const unsigned long long w0 = __hip_atomic_load(rec + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long w1 = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long w2 = __hip_atomic_load(rec + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(w0 & w1 & w2 & TAG_V)) return false;
+    c.leaves[0] = (uint32_t)(w0 & 0x7FFFFFFFull);
+    c.leaves[1] = (uint32_t)((w0 >> 31) & 0x7FFFFFFFull);
+    c.leaves[2] = (uint32_t)(w1 & 0x7FFFFFFFull);
+    c.leaves[3] = (uint32_t)((w1 >> 31) & 0x7FFFFFFFull);
+    const uint32_t m = (uint32_t)w2;
+    c.tt = m & 0xFFFF;
+    c.value = (m >> 16) & 0x7FF;
+    c.nLeaves = (m >> 27) & 7;
+    c.used = (m >> 30) & 1;
+    return true;
+}
+
+__device__ __forceinline__ void store_tagged(unsigned long long *rec, uint32_t l0, uint32_t l1, uint32_t l2,
+                                             uint32_t l3, uint32_t meta) {
+    __hip_atomic_store(rec + 0, (unsigned long long)l0 | (unsigned long long)l1 << 31 | TAG_V, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(rec + 1, (unsigned long long)l2 | (unsigned long long)l3 << 31 | TAG_V, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(rec + 2, (unsigned long long)meta | TAG_V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Cut set of node idx, computed by one whole wavefront (lane = thread lane).
-__device__ __forceinline__ void cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
+// TAGGED: fanin cut sets are polled in the fence-free format and the result is
+// written in it (tcuts, 24 bytes per slot); otherwise plain 20-byte records.
+// Returns false if a poll exceeded its bound (*err set), true otherwise.
+template <bool TAGGED>
+__device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
                                          const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
-                                         uint32_t *cuts) {
+                                         uint32_t *cuts, unsigned long long *tcuts, uint32_t *err) {
     const uint32_t l0 = f0[idx], l1 = f1[idx];
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
     DCut a, b;
-    unpack_cut(cuts + ((size_t)in0 * NCUT + (lane >> 3)) * 5, a);
-    unpack_cut(cuts + ((size_t)in1 * NCUT + (lane & 7)) * 5, b);
+    if (TAGGED) {
+        const unsigned long long *ra = tcuts + ((size_t)in0 * NCUT + (lane >> 3)) * 3;
+        const unsigned long long *rb = tcuts + ((size_t)in1 * NCUT + (lane & 7)) * 3;
+        uint32_t spins = 0;
+        for (;;) {
+            const bool ok = load_tagged(ra, a) && load_tagged(rb, b);
+            if (__ballot(!ok) == 0) break;
+            if (++spins > DF_SPIN_LIMIT ||
+                ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                if (lane == 0) atomicOr(err, 4u);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    } else {
+        unpack_cut(cuts + ((size_t)in0 * NCUT + (lane >> 3)) * 5, a);
+        unpack_cut(cuts + ((size_t)in1 * NCUT + (lane & 7)) * 5, b);
+    }
     // ---- phase 1: every pair in parallel
     int status = C_SKIP, shrunk = 0;
     DCut m, mm;
@@ -361,14 +414,18 @@ __device__ __forceinline__ void cut_node(const int idx, const int lane, const ui
     }
     // ---- phase 3: lanes 0..7 store the slots
     if (lane < NCUT) {
-        uint32_t *o = cuts + ((size_t)idx * NCUT + lane) * 5;
         const bool u = s.used;
-        o[0] = (u && s.n > 0) ? s.l0 : 0u;
-        o[1] = (u && s.n > 1) ? s.l1 : 0u;
-        o[2] = (u && s.n > 2) ? s.l2 : 0u;
-        o[3] = (u && s.n > 3) ? s.l3 : 0u;
-        o[4] = u ? ((s.tt & 0xFFFF) | (s.value & 0x7FF) << 16 | (uint32_t)(s.n & 7) << 27 | 1u << 30) : 0u;
+        const uint32_t o0 = (u && s.n > 0) ? s.l0 : 0u, o1 = (u && s.n > 1) ? s.l1 : 0u;
+        const uint32_t o2 = (u && s.n > 2) ? s.l2 : 0u, o3 = (u && s.n > 3) ? s.l3 : 0u;
+        const uint32_t meta = u ? ((s.tt & 0xFFFF) | (s.value & 0x7FF) << 16 | (uint32_t)(s.n & 7) << 27 | 1u << 30) : 0u;
+        if (TAGGED) {
+            store_tagged(tcuts + ((size_t)idx * NCUT + lane) * 3, o0, o1, o2, o3, meta);
+        } else {
+            uint32_t *o = cuts + ((size_t)idx * NCUT + lane) * 5;
+            o[0] = o0; o[1] = o1; o[2] = o2; o[3] = o3; o[4] = meta;
+        }
     }
+    return true;
 }
 
 __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_t count,
@@ -377,7 +434,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_
                                                            const int32_t *__restrict__ nRef, uint32_t *cuts) {
     const int k = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     if (k >= count) return;    // uniform per wavefront
-    cut_node(start + k, threadIdx.x & 63, f0, f1, nRef, cuts);
+    cut_node<false>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
 }
 
 // All levels in one cooperative launch: the wavefronts of the grid stride over
@@ -391,8 +448,59 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t 
     const int gw = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     for (int l = l0; l < l1; l++) {
         const int s = starts[l], e = starts[l + 1];
-        for (int v = s + gw; v < e; v += nw) cut_node(v, threadIdx.x & 63, f0, f1, nRef, cuts);
+        for (int v = s + gw; v < e; v += nw) cut_node<false>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
         grid.sync();
+    }
+}
+
+// Dataflow schedule: persistent wavefronts take nodes in level-major id order
+// from a ticket counter and wait only for their two fanins' cut sets, polled
+// in the fence-free record format; no level waits for the whole previous
+// level.  Progress: the smallest unfinished ticket has every fanin finished
+// (fanin ids are smaller) and is held by a running wavefront.  Every wait is
+// bounded; on overrun *err is set and all wavefronts stop.
+__global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_dataflow(int32_t first, int32_t last,
+                                                              const uint32_t *__restrict__ f0,
+                                                              const uint32_t *__restrict__ f1,
+                                                              const int32_t *__restrict__ nRef,
+                                                              unsigned long long *tcuts, uint32_t *ticket,
+                                                              uint32_t *err) {
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(ticket, 1u);
+        t = (uint32_t)__shfl((int)t, 0);
+        const int v = first + (int)t;
+        if (v >= last) break;
+        if (!cut_node<true>(v, lane, f0, f1, nRef, nullptr, tcuts, err)) break;
+    }
+}
+
+// PI cut sets in the fence-free format (slot 0 trivial, others empty but valid).
+__global__ void k_cuts_pi_tagged(int32_t nPIs, const int32_t *__restrict__ nRef, unsigned long long *tcuts) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nPIs * NCUT) return;
+    const int64_t v = 1 + i / NCUT;
+    const int slot = (int)(i % NCUT);
+    uint32_t meta = 0;
+    if (slot == 0) meta = 0xAAAAu | (1001u << 16) | (1u << 27) | (1u << 30);
+    store_tagged(tcuts + ((size_t)v * NCUT + slot) * 3, slot == 0 ? (uint32_t)v : 0u, 0, 0, 0, meta);
+    (void)nRef;
+}
+
+// Fence-free records back to the plain 5-word records used by evaluation.
+__global__ void k_untag_cuts(int64_t nslots, const unsigned long long *__restrict__ tcuts, uint32_t *cuts) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i < nslots; i += stride) {
+        const unsigned long long w0 = tcuts[3 * i], w1 = tcuts[3 * i + 1], w2 = tcuts[3 * i + 2];
+        uint32_t *o = cuts + 5 * i;
+        if (!(w2 & TAG_V)) { o[0] = o[1] = o[2] = o[3] = o[4] = 0; continue; }
+        o[0] = (uint32_t)(w0 & 0x7FFFFFFFull);
+        o[1] = (uint32_t)((w0 >> 31) & 0x7FFFFFFFull);
+        o[2] = (uint32_t)(w1 & 0x7FFFFFFFull);
+        o[3] = (uint32_t)((w1 >> 31) & 0x7FFFFFFFull);
+        o[4] = (uint32_t)w2;
     }
 }
 
@@ -977,7 +1085,32 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     // entry is one past the last id.
     const int L = (int)levelStart.size() - 1;
     if (L <= 1) return GLS_OK;
-    bool coop = mode != GLS_RW_PERLEVEL_CUTS && L > 16;
+    if (mode == GLS_RW_DATAFLOW_CUTS && L > 16) {
+        int perCU = 0, dev = 0, nCU = 0;
+        GLS_HIP(ctx, hipGetDevice(&dev));
+        GLS_HIP(ctx, hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
+        GLS_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cuts_dataflow, CUTW_BLOCK, 0));
+        const int G = std::max(1, std::min(perCU, 2) * nCU);
+        const size_t nslots = (size_t)n * NCUT;
+        unsigned long long *tcuts = pool_get<unsigned long long>(ctx, nslots * 3);
+        uint32_t *ctr = pool_get<uint32_t>(ctx, 4);
+        if (!tcuts || !ctr) { pool_put(ctx, tcuts); pool_put(ctx, ctr); return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: tagged records"); }
+        GLS_HIP(ctx, hipMemsetAsync(tcuts, 0, sizeof(unsigned long long) * nslots * 3, st));
+        GLS_HIP(ctx, hipMemsetAsync(ctr, 0, sizeof(uint32_t) * 4, st));
+        if (g.nPIs > 0)
+            k_cuts_pi_tagged<<<blocks((int64_t)g.nPIs * NCUT, kBlock), kBlock, 0, st>>>(g.nPIs, d_nref, tcuts);
+        k_cuts_dataflow<<<G, CUTW_BLOCK, 0, st>>>(g.nPIs + 1, n, g.f0, g.f1, d_nref, tcuts, &ctr[0], &ctr[1]);
+        k_untag_cuts<<<grid_for((int64_t)nslots), kBlock, 0, st>>>((int64_t)nslots, tcuts, d_cuts);
+        GLS_HIP(ctx, hipGetLastError());
+        uint32_t herr = 0;
+        int rc = stream_copy(ctx, &herr, &ctr[1], sizeof(uint32_t), hipMemcpyDeviceToHost);
+        pool_put(ctx, tcuts);
+        pool_put(ctx, ctr);
+        if (rc != GLS_OK) return rc;
+        if (herr) return gls_fail(ctx, GLS_ERR_INTERNAL, "cut enumeration: a wait exceeded its bound");
+        return GLS_OK;
+    }
+    bool coop = mode == GLS_RW_COOP_CUTS && L > 16;
     int G = 0;
     if (coop) {
         int perCU = 0, dev = 0, nCU = 0, coopOk = 0;
@@ -1076,7 +1209,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         unsigned long long *d_gain = t.get<unsigned long long>(1);
         if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n); }
         GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 16, st));
-        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & GLS_RW_PERLEVEL_CUTS));
+        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS)));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};
         GLS_TRY(build_table(ctx, t, g, gv, d_err));

@@ -93,10 +93,12 @@ int gls_strash(gls_aig *aig, int32_t flags);
 /* One DAG-aware rewriting pass with 4-input cuts (rw; use_zeros=1 is rw -z /
  * rwz).  Replaces AIGMan::rewrite (src/aig_manager.cu:632-672) and
  * rw::CPUSolver::Rewrite (src/algorithms/rewrite.cu:811-938).  The AIG stays on
- * the device; the result is strashed.  flags: 0, or GLS_RW_PERLEVEL_CUTS to
- * enumerate cuts with one kernel launch per level instead of one cooperative
- * launch (same results; for tests and tuning). */
-#define GLS_RW_PERLEVEL_CUTS 1
+ * the device; the result is strashed.  flags select how cut enumeration is
+ * scheduled (the results are identical): 0 = one kernel launch per level,
+ * GLS_RW_COOP_CUTS = one cooperative launch with a grid barrier per level,
+ * GLS_RW_DATAFLOW_CUTS = persistent wavefronts that wait only for their fanins. */
+#define GLS_RW_COOP_CUTS 2
+#define GLS_RW_DATAFLOW_CUTS 4
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Info about the last gls_strash / gls_rewrite on this handle. */

@@ -94,14 +94,16 @@ def test_rewrite_parity(gpu_ctx, name, spec):
 
 @pytest.mark.parametrize("name,spec", CASES[1:], ids=[c[0] for c in CASES[1:]])
 def test_cut_launch_modes_agree(gpu_ctx, name, spec):
-    """One cooperative launch over all levels == one launch per level."""
+    """One launch per level, one cooperative launch and the dataflow schedule agree."""
+    import culs_amd
     log("cut launch modes: %s" % name)
     a = make(gpu_ctx, spec)
     a.strash()
-    b = a.copy()
+    b, c = a.copy(), a.copy()
     a.rewrite()
-    b.rewrite(per_level_cuts=True)
-    assert same(a.arrays(), b.arrays())
+    b.rewrite(cut_mode=culs_amd.GLS_RW_DATAFLOW_CUTS)
+    c.rewrite(cut_mode=culs_amd.GLS_RW_COOP_CUTS)
+    assert same(a.arrays(), b.arrays()) and same(a.arrays(), c.arrays())
 
 
 def test_rewrite_deterministic(gpu_ctx):
