@@ -12,7 +12,7 @@ Also printed in the same JSON line:
                 same generator, in the same unit;
   qor           AND counts, depth, and a simulation equivalence check.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nands N] [--cpu-sample N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nands N] [--cpu-sample N] [--cut-mode M]
 """
 import argparse
 import json
@@ -27,6 +27,13 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per AND per rw pass: 3*K*s_c + 64 with K = 8 cuts of s_c = 20 B
 K_CUTS, CUT_BYTES = 8, 20
 B_RW_PER_AND = 3 * K_CUTS * CUT_BYTES + 64
+# cut enumeration per AND: two fanin cut sets read, its own written (3*K*s_c),
+# plus its two fanin literals and its reference count (12 B)
+B_CUT_PER_AND = 3 * K_CUTS * CUT_BYTES + 12
+# HBM bytes per k_cuts_level launch on this workload (FETCH_SIZE + WRITE_SIZE,
+# rocprofv3 --pmc, scaled by the counters' reading for a plain copy kernel);
+# None until measured
+TRAFFIC_BYTES_PER_LAUNCH = None
 
 
 def log(msg):
@@ -44,8 +51,10 @@ def main():
     ap.add_argument("--window", type=int, default=50_000)
     ap.add_argument("--p", type=float, default=0.001)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000,
-                    help="ANDs of the CPU oracle sample (same generator)")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="ANDs of the CPU oracle sample (same generator); 0 = the whole workload")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (torch) 5 times, to scale HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
@@ -56,7 +65,16 @@ def main():
               "(sharding across GPUs is by independent AIGs, not built here); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if a.copy_calibration:
+        import torch
+        x = torch.empty(a.copy_calibration << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel(), "copy_bytes_written": y.numel(), "copies": 5}))
+        return
+
     import culs_amd
 
     ctx = culs_amd.Context(0)
@@ -99,20 +117,24 @@ def main():
     n_out = out.stats().nAnds
     depth_out = out.stats().levels
 
-    # dominant kernel: per-level cut enumeration vs evaluation (event-timed phases)
+    # dominant kernel: the per-level cut enumeration kernel (k_cuts_level).  Every
+    # 16th launch carries HIP events on its own dispatch (the context stream), so
+    # the average is the kernel's execution time as rocprofv3 reports it.
+    ks = sum(i["kernel_samples"] for i in infos)
+    kms = sum(i["kernel_sample_ms"] for i in infos)
+    knodes = sum(i["kernel_sample_nodes"] for i in infos)
     nlev = max(1, info["levels_enumerated"])
-    cand = {
-        "cut_enumeration": (info["cuts_ms"], nlev, n_in * (3 * K_CUTS * CUT_BYTES + 12) / nlev),
-        "evaluation": (info["eval_ms"], 1, n_in * (K_CUTS * CUT_BYTES + 16)),
-    }
-    dom = max(cand, key=lambda k: cand[k][0])
-    ms, launches, bytes_per_launch = cand[dom]
-    avg_launch_s = ms / 1e3 / launches
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "launches": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                "bytes_per_launch": int(bytes_per_launch),
+    if ks > 0 and kms > 0:
+        avg_launch_s = kms / 1e3 / ks
+        bytes_per_launch = knodes * B_CUT_PER_AND / ks
+        achieved = knodes * B_CUT_PER_AND / (kms / 1e3) / 1e9
+    else:
+        avg_launch_s, bytes_per_launch, achieved = 0.0, n_in * B_CUT_PER_AND / nlev, 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+                "kernel": "k_cuts_level", "launches_per_step": nlev, "timed_launches": ks,
+                "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
+                "bytes_per_node": B_CUT_PER_AND,
                 "pass_alg_GBs": round(n_in * B_RW_PER_AND / (total / a.steps) / 1e9, 3)}
 
     qor = {"and_in": a.nands, "and_strashed": n_in, "and_out": n_out, "depth_in": s.levels, "depth_out": depth_out}
@@ -127,14 +149,15 @@ def main():
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_ffi as O
-        sample = ctx.gen_geometric(a.pis, a.cpu_sample, a.pos, a.window, a.p, a.seed).arrays()
+        ns = a.cpu_sample if a.cpu_sample > 0 else a.nands
+        sample = ctx.gen_geometric(a.pis, ns, a.pos, a.window, a.p, a.seed).arrays()
         ss, _ = O.strash(*sample)
         t = time.perf_counter()
         O.rw(*ss)
         dt = time.perf_counter() - t
         cpu = {"value": round(O.n_ands(ss) / dt / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
-               "sample": "oracle rw on the same generator at %d ANDs (%d after strash), %.1f s" %
-                         (a.cpu_sample, O.n_ands(ss), dt)}
+               "sample": "oracle rw (C, one thread) on the same generator and seed at %d ANDs "
+                         "(%d after strash), %.1f s" % (ns, O.n_ands(ss), dt)}
         log("cpu baseline: %s" % cpu)
 
     res = {

@@ -33,7 +33,9 @@ class OpInfo(ctypes.Structure):
                 ("table_ms", ctypes.c_double), ("eval_ms", ctypes.c_double), ("select_ms", ctypes.c_double),
                 ("build_ms", ctypes.c_double), ("strash_ms", ctypes.c_double),
                 ("candidates", ctypes.c_int64), ("selected", ctypes.c_int64), ("new_nodes", ctypes.c_int64),
-                ("rounds", ctypes.c_int64), ("levels_enumerated", ctypes.c_int64), ("est_gain", ctypes.c_int64)]
+                ("rounds", ctypes.c_int64), ("levels_enumerated", ctypes.c_int64), ("est_gain", ctypes.c_int64),
+                ("kernel_sample_ms", ctypes.c_double), ("kernel_samples", ctypes.c_int64),
+                ("kernel_sample_nodes", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -137,6 +137,8 @@ extern "C" void gls_ctx_destroy(gls_ctx *ctx) {
         ctx->pool.release_all();
         for (auto &e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : ctx->kev)
+            if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -56,6 +56,10 @@ struct gls_ctx {
     uint8_t *d_lib_roots = nullptr;
     hipEvent_t ev[16] = {};
     bool lib_ready = false;
+    // sampled per-launch timing: event pairs attached to kernel dispatches
+    std::vector<hipEvent_t> kev;      // 2 per sample, created on demand
+    std::vector<int64_t> ksamp_units; // units (nodes) of each sampled launch
+    int ksamp_n = 0;
 };
 
 // A device-resident AIG in the AIGER literal convention.

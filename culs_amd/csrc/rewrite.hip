@@ -23,6 +23,8 @@
 #include "dev_common.h"
 #include "rwlib_forest.h"
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
+#include <cstdlib>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -309,18 +311,22 @@ __device__ __forceinline__ void store_tagged(unsigned long long *rec, uint32_t l
     __hip_atomic_store(rec + 2, (unsigned long long)meta | TAG_V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// cut_node record modes
+enum : int { REC_PLAIN = 0, REC_TAGGED = 1 };
+
 // Cut set of node idx, computed by one whole wavefront (lane = thread lane).
-// TAGGED: fanin cut sets are polled in the fence-free format and the result is
-// written in it (tcuts, 24 bytes per slot); otherwise plain 20-byte records.
+// MODE REC_TAGGED: fanin cut sets are polled in the fence-free format and the
+// result is written in it (tcuts, 24 bytes per slot); REC_PLAIN: plain 20-byte
+// records.
 // Returns false if a poll exceeded its bound (*err set), true otherwise.
-template <bool TAGGED>
+template <int MODE>
 __device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
                                          const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
                                          uint32_t *cuts, unsigned long long *tcuts, uint32_t *err) {
     const uint32_t l0 = f0[idx], l1 = f1[idx];
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
     DCut a, b;
-    if (TAGGED) {
+    if (MODE == REC_TAGGED) {
         const unsigned long long *ra = tcuts + ((size_t)in0 * NCUT + (lane >> 3)) * 3;
         const unsigned long long *rb = tcuts + ((size_t)in1 * NCUT + (lane & 7)) * 3;
         uint32_t spins = 0;
@@ -418,7 +424,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         const uint32_t o0 = (u && s.n > 0) ? s.l0 : 0u, o1 = (u && s.n > 1) ? s.l1 : 0u;
         const uint32_t o2 = (u && s.n > 2) ? s.l2 : 0u, o3 = (u && s.n > 3) ? s.l3 : 0u;
         const uint32_t meta = u ? ((s.tt & 0xFFFF) | (s.value & 0x7FF) << 16 | (uint32_t)(s.n & 7) << 27 | 1u << 30) : 0u;
-        if (TAGGED) {
+        if (MODE == REC_TAGGED) {
             store_tagged(tcuts + ((size_t)idx * NCUT + lane) * 3, o0, o1, o2, o3, meta);
         } else {
             uint32_t *o = cuts + ((size_t)idx * NCUT + lane) * 5;
@@ -434,7 +440,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_
                                                            const int32_t *__restrict__ nRef, uint32_t *cuts) {
     const int k = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     if (k >= count) return;    // uniform per wavefront
-    cut_node<false>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
+    cut_node<REC_PLAIN>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
 }
 
 // All levels in one cooperative launch: the wavefronts of the grid stride over
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t 
     const int gw = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     for (int l = l0; l < l1; l++) {
         const int s = starts[l], e = starts[l + 1];
-        for (int v = s + gw; v < e; v += nw) cut_node<false>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
+        for (int v = s + gw; v < e; v += nw) cut_node<REC_PLAIN>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
         grid.sync();
     }
 }
@@ -473,7 +479,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_dataflow(int32_t first, int
         t = (uint32_t)__shfl((int)t, 0);
         const int v = first + (int)t;
         if (v >= last) break;
-        if (!cut_node<true>(v, lane, f0, f1, nRef, nullptr, tcuts, err)) break;
+        if (!cut_node<REC_TAGGED>(v, lane, f0, f1, nRef, nullptr, tcuts, err)) break;
     }
 }
 
@@ -1073,10 +1079,21 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
     return GLS_OK;
 }
 
+// every KSAMPLE_EVERY-th per-level cut launch is timed on its own dispatch
+// (GLS_KERNEL_SAMPLE=0 in the environment turns this off, e.g. under a
+// profiler that collects counters per dispatch)
+static constexpr int KSAMPLE_EVERY = 16;
+static const bool kSampleLaunches = [] {
+    const char *e = getenv("GLS_KERNEL_SAMPLE");
+    return !(e && e[0] == '0');
+}();
+
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
                        int32_t *d_nref, int mode) {
     hipStream_t st = ctx->stream;
     const int32_t n = g.nObjs;
+    ctx->ksamp_n = 0;
+    ctx->ksamp_units.clear();
     GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
     k_nref<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f0, g.f1, g.nPOs, g.pos, d_nref);
@@ -1102,12 +1119,16 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         k_cuts_dataflow<<<G, CUTW_BLOCK, 0, st>>>(g.nPIs + 1, n, g.f0, g.f1, d_nref, tcuts, &ctr[0], &ctr[1]);
         k_untag_cuts<<<grid_for((int64_t)nslots), kBlock, 0, st>>>((int64_t)nslots, tcuts, d_cuts);
         GLS_HIP(ctx, hipGetLastError());
-        uint32_t herr = 0;
-        int rc = stream_copy(ctx, &herr, &ctr[1], sizeof(uint32_t), hipMemcpyDeviceToHost);
+        uint32_t hctr[2] = {0, 0};
+        int rc = stream_copy(ctx, hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost);
         pool_put(ctx, tcuts);
         pool_put(ctx, ctr);
         if (rc != GLS_OK) return rc;
-        if (herr) return gls_fail(ctx, GLS_ERR_INTERNAL, "cut enumeration: a wait exceeded its bound");
+        if (hctr[1]) return gls_fail(ctx, GLS_ERR_INTERNAL, "cut enumeration: a wait exceeded its bound");
+        // every node must have been taken (a wavefront stops once tickets pass the end)
+        if (hctr[0] < (uint32_t)(n - g.nPIs - 1))
+            return gls_fail(ctx, GLS_ERR_INTERNAL, "cut enumeration: %u of %d nodes processed", hctr[0],
+                            n - g.nPIs - 1);
         return GLS_OK;
     }
     bool coop = mode == GLS_RW_COOP_CUTS && L > 16;
@@ -1139,7 +1160,24 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         for (int l = 1; l < L; l++) {
             int32_t s = levelStart[l], e = levelStart[l + 1];
             if (e <= s) continue;
-            k_cuts_level<<<blocks(e - s, CUTW_BLOCK / 64), CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+            const dim3 grid(blocks(e - s, CUTW_BLOCK / 64));
+            if (kSampleLaunches && l % KSAMPLE_EVERY == 0) {
+                // this launch carries events on its own dispatch: their elapsed
+                // time is the kernel's execution time, as a profiler reports it
+                const size_t k = 2 * (size_t)ctx->ksamp_n;
+                while (ctx->kev.size() < k + 2) {
+                    hipEvent_t ev = nullptr;
+                    GLS_HIP(ctx, hipEventCreate(&ev));
+                    ctx->kev.push_back(ev);
+                }
+                hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTW_BLOCK), 0, st, ctx->kev[k], ctx->kev[k + 1], 0,
+                                      s, e - s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
+                                      (const int32_t *)d_nref, d_cuts);
+                ctx->ksamp_units.push_back(e - s);
+                ctx->ksamp_n++;
+            } else {
+                k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+            }
         }
     }
     GLS_HIP(ctx, hipGetLastError());
@@ -1320,6 +1358,16 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->rounds = rounds;
             info->levels_enumerated = (int64_t)levelStart.size() - 2;
             info->est_gain = (int64_t)estGain;
+            double kms = 0;
+            int64_t kunits = 0;
+            for (int i = 0; i < ctx->ksamp_n; i++) {
+                float x = 0;
+                if (hipEventElapsedTime(&x, ctx->kev[2 * i], ctx->kev[2 * i + 1]) == hipSuccess) kms += x;
+                kunits += ctx->ksamp_units[i];
+            }
+            info->kernel_sample_ms = kms;
+            info->kernel_samples = ctx->ksamp_n;
+            info->kernel_sample_nodes = kunits;
         }
     }
     return rc;

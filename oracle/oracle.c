@@ -796,9 +796,25 @@ static void rw_free(RwIn *r) { free(r->f0); free(r->f1); free(r->pos); free(r->n
 static int g_dbg_max_sel = -1;   /* debugging aid: apply at most this many selections */
 void orc_debug_max_sel(int k) { g_dbg_max_sel = k; }
 
+static int rw_impl(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPOs,
+                   const uint32_t *posin, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+                   uint32_t *opos, int64_t *stats, int composeOnly);
+
 int orc_rw(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPOs,
            const uint32_t *posin, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
            uint32_t *opos, int64_t *stats) {
+    return rw_impl(n0, nPIs, f0in, f1in, nPOs, posin, useZeros, cap, of0, of1, opos, stats, 0);
+}
+
+int orc_rw_compose(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPOs,
+                   const uint32_t *posin, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+                   uint32_t *opos, int64_t *stats) {
+    return rw_impl(n0, nPIs, f0in, f1in, nPOs, posin, useZeros, cap, of0, of1, opos, stats, 1);
+}
+
+static int rw_impl(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPOs,
+                   const uint32_t *posin, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+                   uint32_t *opos, int64_t *stats, int composeOnly) {
     npn_init();
     RwIn r;
     rw_prepare(n0, nPIs, f0in, f1in, nPOs, posin, &r);
@@ -868,7 +884,16 @@ int orc_rw(int n0, int nPIs, const uint32_t *f0in, const uint32_t *f1in, int nPO
     for (int i = 0; i < nPOs; i++) cpos[i] = repl[ID(r.pos[i])] ^ CMP(r.pos[i]);
     int ret;
     if (N > cap) ret = -2;
-    else {
+    else if (composeOnly) {
+        memcpy(of0, c0, sizeof(uint32_t) * (size_t)N);
+        memcpy(of1, c1, sizeof(uint32_t) * (size_t)N);
+        memcpy(opos, cpos, sizeof(uint32_t) * (size_t)nPOs);
+        ret = N;
+        if (stats) {
+            stats[0] = nCand; stats[1] = nSel; stats[2] = nNew; stats[3] = N;
+            stats[4] = 0; stats[5] = 0; stats[6] = estGain; stats[7] = 0;
+        }
+    } else {
         int depth;
         ret = orc_strash(N, nPIs, c0, c1, nPOs, cpos, of0, of1, opos, &depth);
         if (stats) {

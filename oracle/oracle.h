@@ -47,6 +47,13 @@ int orc_rw(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs
            const uint32_t *pos, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
            uint32_t *opos, int64_t *stats);
 
+/* The graph orc_rw hands to its final strash: the reordered input with the
+ * selected roots substituted and the new nodes appended (ids n..N-1, fanins
+ * possibly above their own id's level order but acyclic).  Returns N. */
+int orc_rw_compose(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+                   const uint32_t *pos, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
+                   uint32_t *opos, int64_t *stats);
+
 /* One rw pass following the reference's own replacement method (sequential
  * host choice pass) — used for QoR comparison only. */
 int orc_rw_culs(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,

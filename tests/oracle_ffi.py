@@ -113,6 +113,11 @@ def rw(nPIs, f0, f1, pos, use_zeros=False):
     return _rw(lib().orc_rw, nPIs, f0, f1, pos, use_zeros)
 
 
+def rw_compose(nPIs, f0, f1, pos, use_zeros=False):
+    """The composed graph rw hands to its final strash (old ids keep the reordered numbering)."""
+    return _rw(lib().orc_rw_compose, nPIs, f0, f1, pos, use_zeros)
+
+
 def rw_culs(nPIs, f0, f1, pos, use_zeros=False):
     """The reference's replacement method (sequential choice pass) — QoR baseline."""
     return _rw(lib().orc_rw_culs, nPIs, f0, f1, pos, use_zeros)

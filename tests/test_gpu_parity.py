@@ -94,7 +94,7 @@ def test_rewrite_parity(gpu_ctx, name, spec):
 
 @pytest.mark.parametrize("name,spec", CASES[1:], ids=[c[0] for c in CASES[1:]])
 def test_cut_launch_modes_agree(gpu_ctx, name, spec):
-    """One launch per level, one cooperative launch and the dataflow schedule agree."""
+    """One launch per level, one cooperative launch and the dataflow schedules agree."""
     import culs_amd
     log("cut launch modes: %s" % name)
     a = make(gpu_ctx, spec)
