@@ -59,6 +59,12 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
                     help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
+    ap.add_argument("--workload", choices=["rw10m", "batch"], default="rw10m",
+                    help="rw10m: the headline line; batch: 4096 mixed AIGs packed into one arena vs one at a time")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-max", type=int, default=200_000, help="largest AND count of the batch mix")
+    ap.add_argument("--batch-check", type=int, default=256,
+                    help="members compared array by array with their one-at-a-time result")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the path is a single-device kernel pipeline "
@@ -70,13 +76,15 @@ def main():
         x = torch.empty(a.copy_calibration << 20, dtype=torch.uint8, device="cuda").fill_(1)
         y = torch.empty_like(x)
         for _ in range(5):
-            y.copy_(x)
+            torch.bitwise_not(x, out=y)     # one elementwise kernel: reads and writes MIB MiB
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": x.numel(), "copy_bytes_written": y.numel(), "copies": 5}))
         return
 
     import culs_amd
 
+    if a.workload == "batch":
+        return bench_batch(a)
     ctx = culs_amd.Context(0)
     if not ctx.has_device:
         raise SystemExit("no GPU")
@@ -184,6 +192,82 @@ def main():
                                                       "build_ms", "strash_ms", "total_ms")},
         "rw_counts": {k: info[k] for k in ("candidates", "selected", "new_nodes", "rounds", "est_gain")},
         "strash_ms": round(st_info["total_ms"], 3),
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_batch(a):
+    """Batched rw: --batch-count seeded random AIGs (ANDs log-uniform in
+    [1000, --batch-max], PIs = max(16, ANDs/200), POs = PIs/2, fanins uniform over
+    the PIs and the previous 1000 ANDs, seeds 1000+i), strashed, then one rw
+    pass packed into one device arena vs the same pass one AIG at a time.
+    value = total ANDs entering rw / batched rw time."""
+    import numpy as np
+    import culs_amd
+    ctx = culs_amd.Context(0)
+    rng = np.random.default_rng(12345)
+    t0 = time.time()
+    aigs = []
+    for i in range(a.batch_count):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(a.batch_max))))
+        pis = max(16, n // 200)
+        aigs.append(ctx.gen_random(pis, n, max(1, pis // 2), 1000, 1000 + i))
+    log("generated %d AIGs in %.1fs" % (len(aigs), time.time() - t0))
+    ctx.batch_strash(aigs)
+    n_in = sum(x.n_ands for x in aigs)
+    depth = max(x.stats().levels for x in aigs)
+    log("strashed: %d ANDs in total, deepest %d levels" % (n_in, depth))
+
+    def copies():
+        return [x.copy() for x in aigs]
+
+    for _ in range(a.warmup):
+        w = copies()
+        ctx.batch_rewrite(w)
+        for x in w:
+            x.free()
+    times, out = [], None
+    for i in range(a.steps):
+        c = copies()
+        t = time.perf_counter()
+        ctx.batch_rewrite(c)
+        times.append(time.perf_counter() - t)
+        if out is not None:
+            for x in out:
+                x.free()
+        out = c
+        log("batch step %d: %.1f ms" % (i, times[-1] * 1e3))
+    t_batch = sum(times) / len(times)
+    info = out[0].last_op()
+    # the plain loop: the same pass, one AIG at a time (each already on the device)
+    c = copies()
+    t = time.perf_counter()
+    for x in c:
+        x.rewrite()
+    t_loop = time.perf_counter() - t
+    log("one-at-a-time loop: %.1f ms" % (t_loop * 1e3))
+    same_count = all(x.n_ands == y.n_ands for x, y in zip(out, c))
+    idx = np.linspace(0, len(c) - 1, min(a.batch_check, len(c))).astype(int)
+    same_arrays = all(all(np.array_equal(p, q) for p, q in zip(out[k].arrays(), c[k].arrays())) for k in idx)
+    n_out = sum(x.n_ands for x in out)
+    res = {
+        "metric": "batched rw Mnodes/s, %d mixed AIGs" % len(aigs),
+        "value": round(n_in / t_batch / 1e6, 4),
+        "unit": "Mnodes/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3),
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "batched rw over %d seeded random AIGs, ANDs log-uniform in [1000, %d], "
+                               "PIs = max(16, ANDs/200), POs = PIs/2, fanin window 1000, seeds 1000+i"
+                               % (len(aigs), a.batch_max),
+                   "ands_total": n_in, "deepest": depth},
+        "loop": {"value": round(n_in / t_loop / 1e6, 4), "ms": round(t_loop * 1e3, 3)},
+        "speedup_vs_loop": round(t_loop / t_batch, 3),
+        "identical_to_alone": {"and_counts_all": bool(same_count), "arrays_checked": int(len(idx)),
+                               "arrays_equal": bool(same_arrays)},
+        "qor": {"and_in": n_in, "and_out": n_out},
+        "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
+                                                      "build_ms", "strash_ms", "total_ms")},
     }
     print(json.dumps(res), flush=True)
 

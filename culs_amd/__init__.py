@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgls.so")
 
 GLS_LIT_AIGER = 0
 GLS_LIT_CULS = 1
+GLS_ST_CPU = 1
 GLS_RW_COOP_CUTS = 2
 GLS_RW_DATAFLOW_CUTS = 4
 
@@ -61,6 +62,8 @@ _SIGS = {
     "gls_strash": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gls_rewrite": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "gls_aig_last_op": (ctypes.c_int, [_P, ctypes.POINTER(OpInfo)]),
+    "gls_batch_strash": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32]),
+    "gls_batch_rewrite": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "gls_reorder": (ctypes.c_int, [_P, _P]),
     "gls_enumerate_cuts": (ctypes.c_int, [_P, _P]),
     "gls_evaluate": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
@@ -156,6 +159,23 @@ class Context:
                    "gen_geometric")
         return Aig(self, h)
 
+    def _handles(self, aigs):
+        arr = (_P * max(1, len(aigs)))()
+        for i, a in enumerate(aigs):
+            arr[i] = a._h
+        return arr
+
+    def batch_strash(self, aigs):
+        """Strash every AIG of the list in one device pass (results as if alone)."""
+        self.check(lib().gls_batch_strash(self._h, self._handles(aigs), len(aigs), 0), "gls_batch_strash")
+        return aigs
+
+    def batch_rewrite(self, aigs, use_zeros=False, cut_mode=0):
+        """One rw pass over every AIG of the list, packed into one arena."""
+        self.check(lib().gls_batch_rewrite(self._h, self._handles(aigs), len(aigs), 1 if use_zeros else 0, cut_mode),
+                   "gls_batch_rewrite")
+        return aigs
+
     def gen_multiplier(self, nbits, redundant, seed):
         h = _P()
         self.check(lib().gls_gen_multiplier(self._h, nbits, redundant, seed, ctypes.byref(h)), "gen_multiplier")
@@ -205,8 +225,9 @@ class Aig:
     def write_aiger(self, path):
         self.ctx.check(lib().gls_aig_write_aiger(self._h, path.encode()), "gls_aig_write_aiger")
 
-    def strash(self):
-        self.ctx.check(lib().gls_strash(self._h, 0), "gls_strash")
+    def strash(self, cpu=False):
+        """Device strash; cpu=True is `st -c` (host strash without trivial folding)."""
+        self.ctx.check(lib().gls_strash(self._h, GLS_ST_CPU if cpu else 0), "gls_strash")
         return self
 
     def rewrite(self, use_zeros=False, cut_mode=0):

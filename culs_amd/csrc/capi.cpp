@@ -1,5 +1,6 @@
 // C ABI of libgls.so: contexts, AIG handles and the entry points of gls.h.
 #include "gls_internal.h"
+#include <chrono>
 #include "rwlib_forest.h"
 #include <cstdarg>
 #include <cstring>
@@ -246,9 +247,12 @@ int aig_ensure_device(gls_aig *a) {
     GLS_HIP(ctx, hipMemcpyAsync(a->dev.f1, a->h_f1.data(), sizeof(uint32_t) * a->nObjs, hipMemcpyHostToDevice, ctx->stream));
     if (a->nPOs)
         GLS_HIP(ctx, hipMemcpyAsync(a->dev.pos, a->h_pos.data(), sizeof(uint32_t) * a->nPOs, hipMemcpyHostToDevice, ctx->stream));
+    if (a->host_level_valid)
+        GLS_HIP(ctx, hipMemcpyAsync(a->dev.level, a->h_level.data(), sizeof(uint32_t) * a->nObjs,
+                                    hipMemcpyHostToDevice, ctx->stream));
     GLS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     a->dev_valid = true;
-    a->dev_level_valid = false;
+    a->dev_level_valid = a->host_level_valid;
     return GLS_OK;
 }
 
@@ -317,6 +321,8 @@ extern "C" int gls_aig_copy(gls_aig *src, gls_aig **out) {
     gls_aig *a = new_aig(src->ctx, src->nPIs, f0, f1, p);
     a->levels = src->levels;
     a->strashed = src->strashed;
+    a->h_level = src->h_level;
+    a->host_level_valid = src->host_level_valid;
     *out = a;
     return GLS_OK;
 }
@@ -367,11 +373,38 @@ static void adopt(gls_aig *a, DevGraph &out, int32_t depth) {
     a->dev_valid = true;
     a->dev_level_valid = true;
     a->host_valid = false;
+    a->host_level_valid = false;
     a->levels = depth;
+}
+
+// `st -c`: host strash (AIGMan::strash(fCPU = true), strash.cu:322-466)
+static int strash_cpu(gls_aig *a) {
+    GLS_TRY(aig_ensure_host(a));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> f0, f1, pos, lev;
+    int32_t depth = 0, nDangling = 0;
+    GLS_TRY(host_strash_cpu(a->nPIs, a->h_f0, a->h_f1, a->h_pos, f0, f1, pos, lev, depth, nDangling));
+    aig_drop_device(a);
+    a->nObjs = (int32_t)f0.size();
+    a->h_f0.swap(f0);
+    a->h_f1.swap(f1);
+    a->h_pos.swap(pos);
+    a->h_level.swap(lev);
+    a->host_valid = true;
+    a->host_level_valid = true;
+    a->levels = depth;
+    a->strashed = true;
+    gls_op_info info = {};
+    info.total_ms = info.strash_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    a->last = info;
+    return GLS_OK;
 }
 
 extern "C" int gls_strash(gls_aig *a, int32_t flags) {
     if (!a) return GLS_ERR_ARG;
+    if (flags & ~GLS_ST_CPU) return gls_fail(a->ctx, GLS_ERR_ARG, "gls_strash: unknown flags 0x%x", flags);
+    if (flags & GLS_ST_CPU) return strash_cpu(a);
     gls_ctx *ctx = a->ctx;
     GLS_TRY(aig_ensure_device(a));
     GLS_HIP(ctx, hipEventRecord(ctx->ev[14], ctx->stream));
@@ -387,7 +420,6 @@ extern "C" int gls_strash(gls_aig *a, int32_t flags) {
     adopt(a, out, depth);
     a->strashed = true;
     a->last = info;
-    (void)flags;
     return GLS_OK;
 }
 

@@ -78,6 +78,8 @@ struct gls_aig {
     std::vector<uint32_t> h_f0, h_f1, h_pos;  // AIGER convention
     DevGraph dev;                               // device copy (pool memory)
     bool dev_level_valid = false;
+    std::vector<uint32_t> h_level;              // node levels of the host copy
+    bool host_level_valid = false;              // (uploaded with it when valid)
     gls_op_info last = {};
 };
 
@@ -87,6 +89,12 @@ template <class T> T *pool_get(gls_ctx *ctx, size_t n) {
 }
 void pool_put(gls_ctx *ctx, void *p);
 int stream_copy(gls_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
+
+// host strash of `st -c` (cpu_strash.cpp)
+int host_strash_cpu(int32_t nPIs, const std::vector<uint32_t> &f0, const std::vector<uint32_t> &f1,
+                    const std::vector<uint32_t> &pos, std::vector<uint32_t> &of0, std::vector<uint32_t> &of1,
+                    std::vector<uint32_t> &opos, std::vector<uint32_t> &olevel, int32_t &depth,
+                    int32_t &nDangling);
 
 // host <-> device
 int aig_ensure_device(gls_aig *aig);
@@ -103,10 +111,17 @@ int aiger_write(const char *path, int32_t nPIs, const std::vector<uint32_t> &f0,
 
 // device passes (strash.hip / rewrite.hip).  Both consume `in` (the caller
 // releases it) and fill `out` with fresh pool memory, out.level included.
-int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, gls_op_info *info);
-int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart);
+// tagIn / tagOut (optional): a per-node tag carried to the surviving node
+// (*tagOut is allocated from the pool, out.nObjs entries; the caller frees it).
+// tagDepth (optional, zeroed by the caller, indexed by tag): the depth each
+// tag's nodes alone would report, i.e. the largest wave among its live nodes.
+int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, gls_op_info *info,
+               const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr, uint32_t *tagDepth = nullptr);
+int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart,
+                const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr);
 int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
-                gls_op_info *info);
+                gls_op_info *info, const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr,
+                uint32_t *tagDepth = nullptr);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
                        uint32_t *d_cuts, int32_t *d_nref, int mode);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,

@@ -101,8 +101,8 @@ int timeHandler(Shell &sh, const std::vector<std::string> &) {
 int strashHandler(Shell &sh, const std::vector<std::string> &v) {
     if (!flags_ok(v, "cv")) return 1;
     if (!need_aig(sh, "strash")) return 0;
-    if (has_flag(v, "-c")) printf("st -c: this build strashes on the device only; running the device strash\n");
-    int rc = timed(sh, [&] { return gls_strash(sh.aig, 0); });
+    const int32_t flags = has_flag(v, "-c") ? GLS_ST_CPU : 0;
+    int rc = timed(sh, [&] { return gls_strash(sh.aig, flags); });
     if (rc != GLS_OK) report_error(sh, "strash", rc);
     return 0;
 }

@@ -885,7 +885,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_build(int32_t n, int32_t nCand, 
                                                        GraphView g, Lib L, const uint32_t *__restrict__ cuts,
                                                        const int4 *__restrict__ best, const uint8_t *__restrict__ state,
                                                        const uint32_t *__restrict__ off, uint32_t *cf0, uint32_t *cf1,
-                                                       uint32_t *repl, uint32_t *err) {
+                                                       uint32_t *repl, uint32_t *err, uint32_t *ctag) {
     __shared__ int32_t s_match[NMATCH * EVAL_BLOCK], s_mlev[NMATCH * EVAL_BLOCK];
     int64_t c = blockIdx.x * (int64_t)EVAL_BLOCK + threadIdx.x;
     if (c >= nCand || state[c] != ST_SELECTED) return;
@@ -898,6 +898,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_build(int32_t n, int32_t nCand, 
     uint32_t next = (uint32_t)n + off[c];
     for (int j = 4; j <= pc.rt; j++) {
         if (!((pc.addMask >> j) & 1)) continue;
+        if (ctag) ctag[next] = ctag[id];   // a new node carries its root's tag
         uint32_t wd = lib_word(L, pc.cls, j - 4);
         int a = wd & 63, bb = (wd >> 6) & 63;
         if (match[a] < 0 || match[bb] < 0) {  // cannot happen for a consistent plan
@@ -951,11 +952,13 @@ __global__ void k_reorder_ids(int32_t m, const uint32_t *__restrict__ sortedIds,
 
 __global__ void k_reorder_emit(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                                const uint32_t *__restrict__ f1, const uint32_t *__restrict__ level,
-                               const uint32_t *__restrict__ newId, uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
+                               const uint32_t *__restrict__ newId, uint32_t *of0, uint32_t *of1, uint32_t *olevel,
+                               const uint32_t *__restrict__ tagIn, uint32_t *tagOut) {
     int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (v >= n) return;
     uint32_t nv = newId[v];
     olevel[nv] = level[v];
+    if (tagOut) tagOut[nv] = tagIn[v];
     if (v <= nPIs) { of0[nv] = 0; of1[nv] = 0; return; }
     uint32_t a = 2u * newId[LID(f0[v])] | LC(f0[v]);
     uint32_t b = 2u * newId[LID(f1[v])] | LC(f1[v]);
@@ -1028,7 +1031,8 @@ struct Timer {
 
 } // namespace
 
-int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart) {
+int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart,
+                const uint32_t *tagIn, uint32_t **tagOut) {
     const int32_t n = in.nObjs, m = n - 1;
     hipStream_t st = ctx->stream;
     Tmp t{ctx};
@@ -1037,6 +1041,12 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
     if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: allocation");
     uint32_t maxLevel = 0;
     GLS_TRY(dev_graph_alloc(ctx, out, n, in.nPIs, in.nPOs));
+    uint32_t *tags = nullptr;
+    if (tagIn && tagOut) {
+        tags = pool_get<uint32_t>(ctx, (size_t)n);
+        if (!tags) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: tags"); }
+        *tagOut = tags;
+    }
     if (m > 0) {
         k_gather_level<<<blocks(m, kBlock), kBlock, 0, st>>>(m, in.level, keys);
         k_iota1<<<blocks(m, kBlock), kBlock, 0, st>>>(m, ids);
@@ -1061,7 +1071,7 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
         levelStart.assign(maxLevel + 2, 0);
         GLS_HIP(ctx, hipMemcpyAsync(levelStart.data(), d_starts, sizeof(int32_t) * (maxLevel + 1), hipMemcpyDeviceToHost, st));
         k_reorder_emit<<<blocks(n, kBlock), kBlock, 0, st>>>(n, in.nPIs, in.f0, in.f1, in.level, newId, out.f0,
-                                                              out.f1, out.level);
+                                                              out.f1, out.level, tagIn, tags);
         if (in.nPOs > 0)
             k_reorder_pos<<<blocks(in.nPOs, kBlock), kBlock, 0, st>>>(in.nPOs, in.pos, newId, out.pos);
         GLS_HIP(ctx, hipGetLastError());
@@ -1073,6 +1083,7 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
         GLS_HIP(ctx, hipMemsetAsync(out.level, 0, sizeof(uint32_t) * n, st));
         if (in.nPOs > 0)
             GLS_HIP(ctx, hipMemcpyAsync(out.pos, in.pos, sizeof(uint32_t) * in.nPOs, hipMemcpyDeviceToDevice, st));
+        if (tags) GLS_HIP(ctx, hipMemcpyAsync(tags, tagIn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
         GLS_HIP(ctx, hipStreamSynchronize(st));
         levelStart.assign(2, n);
     }
@@ -1225,14 +1236,21 @@ int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const 
 }
 
 int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
-                gls_op_info *info) {
+                gls_op_info *info, const uint32_t *tagIn, uint32_t **tagOut, uint32_t *tagDepth) {
     GLS_TRY(ctx_ensure_library(ctx));
     hipStream_t st = ctx->stream;
     Timer tm{ctx};
     tm.mark(); // 0
     DevGraph g;
     std::vector<int32_t> levelStart;
-    GLS_TRY(dev_reorder(ctx, in, g, levelStart));
+    const bool tagged = tagIn && tagOut;
+    uint32_t *tagR = nullptr;   // tags in the reordered numbering (freed on every return)
+    struct TagGuard {
+        gls_ctx *c;
+        uint32_t **p;
+        ~TagGuard() { if (*p) pool_put(c, *p); }
+    } tagGuard{ctx, &tagR};
+    GLS_TRY(dev_reorder(ctx, in, g, levelStart, tagged ? tagIn : nullptr, tagged ? &tagR : nullptr));
     tm.mark(); // 1
     const int32_t n = g.nObjs, nPIs = g.nPIs, nPOs = g.nPOs;
     int rc = GLS_OK;
@@ -1328,9 +1346,16 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         const int32_t N = n + (int32_t)totalNew;
         DevGraph comb;
         GLS_TRY(dev_graph_alloc(ctx, comb, N, nPIs, nPOs));
+        uint32_t *tagC = nullptr;   // tags of the composed graph
+        if (tagged) {
+            tagC = t.get<uint32_t>(N);
+            if (!tagC) { dev_graph_free(ctx, g); dev_graph_free(ctx, comb); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: tags"); }
+            GLS_HIP(ctx, hipMemcpyAsync(tagC, tagR, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        }
         if (nCand > 0 && totalNew + nSel > 0)
             k_build<<<blocks(nCand, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, (int32_t)nCand, candIds, gv, L, cuts, best,
-                                                                       state, newOff, comb.f0, comb.f1, repl, d_err);
+                                                                       state, newOff, comb.f0, comb.f1, repl, d_err,
+                                                                       tagC);
         k_compose<<<grid_for(N), kBlock, 0, st>>>(n, N, nPIs, g.f0, g.f1, repl, comb.f0, comb.f1);
         if (nPOs > 0) k_compose_pos<<<blocks(nPOs, kBlock), kBlock, 0, st>>>(nPOs, g.pos, repl, comb.pos);
         GLS_HIP(ctx, hipGetLastError());
@@ -1339,7 +1364,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         if (herr) { dev_graph_free(ctx, g); dev_graph_free(ctx, comb); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr); }
         tm.mark(); // 6
         dev_graph_free(ctx, g);
-        rc = dev_strash(ctx, comb, out, depth, nullptr);
+        rc = dev_strash(ctx, comb, out, depth, nullptr, tagC, tagged ? tagOut : nullptr, tagged ? tagDepth : nullptr);
         dev_graph_free(ctx, comb);
         tm.mark(); // 7
         GLS_HIP(ctx, hipStreamSynchronize(st));

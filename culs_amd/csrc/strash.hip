@@ -333,7 +333,8 @@ __device__ __forceinline__ uint32_t final_lit(uint32_t lit, int32_t nPIs, const 
 __global__ void k_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ sorted,
                        const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
                        const unsigned long long *__restrict__ rl, const uint32_t *__restrict__ newId,
-                       uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
+                       uint32_t *of0, uint32_t *of1, uint32_t *olevel, const uint32_t *__restrict__ tagIn,
+                       uint32_t *tagOut) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = tid; p < m; p += stride) {
         uint32_t v = sorted[p];
@@ -343,6 +344,7 @@ __global__ void k_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ sor
         of0[o] = a;
         of1[o] = b;
         olevel[o] = (uint32_t)(rl[v] >> 32);
+        if (tagOut) tagOut[o] = tagIn[v];
     }
 }
 
@@ -398,7 +400,15 @@ static int run_sweep(gls_ctx *ctx, SweepCtl *d_ctl, uint32_t first, F launch, Sw
     return GLS_OK;
 }
 
-int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOut, gls_op_info *info) {
+__global__ void k_tag_depth(int32_t n, int32_t nPIs, const uint32_t *__restrict__ wave,
+                            const uint32_t *__restrict__ tag, uint32_t *tagDepth) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + tid; v < n; v += stride)
+        if (wave[v]) atomicMax(&tagDepth[tag[v]], wave[v]);   // dangling nodes keep wave 0
+}
+
+int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOut, gls_op_info *info,
+               const uint32_t *tagIn, uint32_t **tagOut, uint32_t *tagDepth) {
     const int32_t n = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs;
     hipStream_t st = ctx->stream;
     Tmp t{ctx};
@@ -493,6 +503,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
         return gls_fail(ctx, GLS_ERR_ARG, "strash: %u of %u AND nodes unreachable in level order (cycle)",
                         nAnds - nDead - processed, nAnds - nDead);
 
+    if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(n, nPIs, wave, tagIn, tagDepth);
     // 4. survivors in (wave, id) order
     k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, slotOf, ht, surv);
     GLS_HIP(ctx, hipGetLastError());
@@ -521,10 +532,17 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
         GLS_HIP(ctx, rocprim::radix_sort_pairs(tmp3, tb3, survWave, sortWave, survIds, sortIds, (size_t)m, 0u, endBit, st));
         sorted = sortIds;
     }
+    uint32_t *tags = nullptr;
+    if (tagIn && tagOut) {
+        tags = pool_get<uint32_t>(ctx, (size_t)nNew);
+        if (!tags) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "strash: tags"); }
+        GLS_HIP(ctx, hipMemcpyAsync(tags, tagIn, sizeof(uint32_t) * (nPIs + 1), hipMemcpyDeviceToDevice, st));
+        *tagOut = tags;
+    }
     k_new_ids<<<grid_for(m + nPIs + 1), kBlock, 0, st>>>(n, nPIs, (int32_t)m, sorted, newId);
     if (m > 0)
         k_emit<<<grid_for(m), kBlock, 0, st>>>((int32_t)m, nPIs, sorted, in.f0, in.f1, rl, newId,
-                                                out.f0, out.f1, out.level);
+                                                out.f0, out.f1, out.level, tagIn, tags);
     if (nPOs > 0) k_map_pos<<<grid_for(nPOs), kBlock, 0, st>>>(nPOs, nPIs, in.pos, rl, newId, out.pos);
     GLS_HIP(ctx, hipGetLastError());
     GLS_HIP(ctx, hipStreamSynchronize(st));

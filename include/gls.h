@@ -92,7 +92,12 @@ int gls_aig_write_aiger(gls_aig *aig, const char *path);
 /* ---- the hot path ----------------------------------------------------- */
 /* Structural hashing + dangling-node removal on the device.  Replaces
  * Aig::strash (src/aig/strash.cuh:12-14, strash.cu:190-314) behind
- * AIGMan::strash(fCPU=false) (strash.cu:317).  flags: reserved, pass 0. */
+ * AIGMan::strash(fCPU=false) (strash.cu:317).
+ * flags = GLS_ST_CPU selects the host strash of `st -c`
+ * (AIGMan::strash(fCPU=true), strash.cu:322-466): dangling removal and
+ * id-order hashing without trivial-case folding, on the host (also in a
+ * host-only context); levels = the largest node delay, -1 if no AND remains. */
+#define GLS_ST_CPU 1
 int gls_strash(gls_aig *aig, int32_t flags);
 
 /* One DAG-aware rewriting pass with 4-input cuts (rw; use_zeros=1 is rw -z /
@@ -108,6 +113,18 @@ int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Info about the last gls_strash / gls_rewrite on this handle. */
 int gls_aig_last_op(gls_aig *aig, gls_op_info *out);
+
+/* ---- batched passes ---------------------------------------------------- */
+/* No counterpart in the reference, which runs one AIG per process through
+ * AIGMan::strash / AIGMan::rewrite (aig_manager.cu:632-672).  The `count`
+ * AIGs (distinct handles of ctx) are packed into one device arena — a
+ * disjoint union with CSR offsets — rewritten by one pass and split back in
+ * place.  Each AIG's result is bit-identical to gls_strash / gls_rewrite on it
+ * alone.  gls_batch_rewrite strashes members that are not strashed first (as
+ * gls_rewrite does); flags as in gls_rewrite.  gls_aig_last_op of every
+ * member then reports the batch as a whole. */
+int gls_batch_strash(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t flags);
+int gls_batch_rewrite(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t use_zeros, int32_t flags);
 
 /* ---- stage-level entry points (parity tests and tools) ----------------- */
 /* Level-major renumbering of a strashed AIG as done before rw

@@ -6,6 +6,7 @@
  * it.  The product never links against it.
  *
  * Reference files restated (paths relative to the CULS tree):
+ *   strash -c         src/aig/strash.cu:322-466  (host strash of `st -c`)
  *   strash            src/aig/strash.cu:175-314  (GPU strash: dangling removal,
  *                     level waves, trivial folding, re-id by tentative id),
  *                     src/hash_table/hash_table.h:422-449 (key, trivial rules)
@@ -260,6 +261,48 @@ int orc_strash(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs
         opos[i] = 2u * (uint32_t)newId[ID(l)] | CMP(l);
     }
     free(newId); free(res); free(surv); free(ord); free(wave); free(live);
+    if (pDepth) *pDepth = depth;
+    return next;
+}
+
+/* ------------------------------------------------------------------ */
+/* Host strash of `st -c` (AIGMan::strash(fCPU=true), strash.cu:322-466)*/
+/* ------------------------------------------------------------------ */
+/* The live set is computed as the transitive fanin of the POs (the      */
+/* reference removes zero-fanout nodes recursively, :360-383; both give  */
+/* the same set).  Nodes are visited in id order; no trivial folding.    */
+/* Fanins are ordered by the reference's own literal values, where lit 0 */
+/* is TRUE (common.h:285-286), i.e. the AIGER constant literal flipped.  */
+static uint32_t culs_val(uint32_t l) { return l < 2 ? 1u - l : l; }
+
+int orc_strash_cpu(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+                   const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos, uint32_t *olevel,
+                   int *pDepth) {
+    uint8_t *live = calloc((size_t)n, 1);
+    for (int i = 0; i < nPOs; i++) live[ID(pos[i])] = 1;
+    for (int v = n - 1; v > nPIs; v--)          /* ids are topological: one reverse sweep */
+        if (live[v]) { live[ID(f0[v])] = 1; live[ID(f1[v])] = 1; }
+    uint32_t *newId = malloc(sizeof(uint32_t) * (size_t)n);
+    for (int v = 0; v <= nPIs && v < n; v++) { newId[v] = (uint32_t)v; of0[v] = of1[v] = 0; olevel[v] = 0; }
+    Map m; map_init(&m, n);
+    int next = nPIs + 1, depth = -1;
+    for (int v = nPIs + 1; v < n; v++) {
+        if (!live[v]) continue;
+        uint32_t a = 2u * newId[ID(f0[v])] + CMP(f0[v]);
+        uint32_t b = 2u * newId[ID(f1[v])] + CMP(f1[v]);
+        if (culs_val(a) > culs_val(b)) { uint32_t t = a; a = b; b = t; }
+        int32_t id = map_put(&m, and_key(a, b), next);
+        newId[v] = (uint32_t)id;
+        if (id == next) {
+            of0[next] = a; of1[next] = b;
+            uint32_t la = olevel[ID(a)], lb = olevel[ID(b)];
+            olevel[next] = 1u + (la > lb ? la : lb);
+            if ((int)olevel[next] > depth) depth = (int)olevel[next];
+            next++;
+        }
+    }
+    for (int i = 0; i < nPOs; i++) opos[i] = 2u * newId[ID(pos[i])] + CMP(pos[i]);
+    map_free(&m); free(newId); free(live);
     if (pDepth) *pDepth = depth;
     return next;
 }

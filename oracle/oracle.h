@@ -26,6 +26,13 @@ void orc_npn_tables(uint8_t *phase, uint8_t *perm, uint8_t *cls, uint8_t *perms4
 int orc_strash(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
                const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos, int *pDepth);
 
+/* Host strash of `st -c`: dangling removal, id-order hashing, no trivial
+ * folding.  olevel receives node delays; *pDepth = largest delay, -1 if no
+ * AND remains.  Returns nObjs of the result. */
+int orc_strash_cpu(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
+                   const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos, uint32_t *olevel,
+                   int *pDepth);
+
 /* Level-major renumbering (ties by id) of an AIG whose ids are topological. */
 int orc_reorder(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
                 const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos,

@@ -68,6 +68,19 @@ def strash(nPIs, f0, f1, pos):
     return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy()), d.value
 
 
+def strash_cpu(nPIs, f0, f1, pos):
+    """`st -c`: host strash without trivial folding -> (graph, levels, depth)."""
+    f0, f1, pos = u32(f0), u32(f1), u32(pos)
+    n = len(f0)
+    o0, o1, ol = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    op = np.zeros(max(len(pos), 1), np.uint32)
+    d = ctypes.c_int(0)
+    r = lib().orc_strash_cpu(n, nPIs, P(f0), P(f1), len(pos), P(pos), P(o0), P(o1), P(op), P(ol), ctypes.byref(d))
+    if r < 0:
+        raise RuntimeError("orc_strash_cpu failed %d" % r)
+    return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy()), ol[:r].copy(), d.value
+
+
 def reorder(nPIs, f0, f1, pos):
     f0, f1, pos = u32(f0), u32(f1), u32(pos)
     n = len(f0)

@@ -38,6 +38,14 @@ def test_host_only_context_refuses_device_work(host_ctx):
         a.rewrite()
 
 
+def test_batch_needs_a_device(host_ctx):
+    a = host_ctx.gen_random(8, 50, 4, 20, 1)
+    with pytest.raises(GlsError):
+        host_ctx.batch_rewrite([a])
+    with pytest.raises(GlsError):
+        host_ctx.batch_strash([a])
+
+
 def test_aiger_round_trip(host_ctx, tmp_path):
     a = host_ctx.gen_multiplier(6, 0.1, 3)
     p = str(tmp_path / "m.aig")

@@ -106,6 +106,22 @@ def test_cut_launch_modes_agree(gpu_ctx, name, spec):
     assert same(a.arrays(), b.arrays()) and same(a.arrays(), c.arrays())
 
 
+@pytest.mark.parametrize("name,spec", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_rewrite_after_host_strash(gpu_ctx, name, spec):
+    """`st -c; rw`: rw on the host strash's output (trivial nodes kept)."""
+    log("st -c; rw: %s" % name)
+    a = make(gpu_ctx, spec)
+    g = a.arrays()
+    a.strash(cpu=True)
+    s = a.arrays()
+    want_s, _, depth = O.strash_cpu(*g)
+    assert same(s, want_s) and a.stats().levels == depth
+    a.rewrite()
+    want, st = O.rw(*s)
+    assert same(a.arrays(), want)
+    assert O.sim_cmp(g, a.arrays()) == -1
+
+
 def test_rewrite_deterministic(gpu_ctx):
     log("rw determinism")
     outs = []
