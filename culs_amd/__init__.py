@@ -62,6 +62,7 @@ _SIGS = {
     "gls_strash": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gls_rewrite": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "gls_aig_last_op": (ctypes.c_int, [_P, ctypes.POINTER(OpInfo)]),
+    "gls_balance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_strash": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_rewrite": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "gls_reorder": (ctypes.c_int, [_P, _P]),
@@ -236,6 +237,11 @@ class Aig:
         launch, GLS_RW_DATAFLOW_CUTS fanin-driven; the results are identical."""
         flags = cut_mode
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
+        return self
+
+    def balance(self, update_level=True):
+        """`b` (update_level: strash afterwards, as the command does; False is `gb`)."""
+        self.ctx.check(lib().gls_balance(self._h, 1, 1 if update_level else 0), "gls_balance")
         return self
 
     def last_op(self):

@@ -439,6 +439,27 @@ extern "C" int gls_rewrite(gls_aig *a, int32_t use_zeros, int32_t flags) {
     return GLS_OK;
 }
 
+extern "C" int gls_balance(gls_aig *a, int32_t sort_dec_id, int32_t update_level) {
+    if (!a) return GLS_ERR_ARG;
+    gls_ctx *ctx = a->ctx;
+    if (sort_dec_id != 1)
+        return gls_fail(ctx, GLS_ERR_ARG, "balance: only sort_dec_id = 1 (level, then literal) is supported");
+    GLS_TRY(aig_ensure_device(a));
+    DevGraph out;
+    gls_op_info info = {};
+    GLS_TRY(dev_balance(ctx, a->dev, out, &info));
+    adopt(a, out, -1);
+    a->dev_level_valid = false;   // levels are not computed by the balance itself
+    a->strashed = false;
+    a->last = info;
+    if (update_level) {
+        const double ms = info.total_ms;
+        GLS_TRY(gls_strash(a, 0));
+        a->last.total_ms += ms;
+    }
+    return GLS_OK;
+}
+
 extern "C" int gls_aig_last_op(gls_aig *a, gls_op_info *out) {
     if (!a || !out) return GLS_ERR_ARG;
     *out = a->last;

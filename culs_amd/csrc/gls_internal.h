@@ -122,6 +122,7 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
 int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
                 gls_op_info *info, const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr,
                 uint32_t *tagDepth = nullptr);
+int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
                        uint32_t *d_cuts, int32_t *d_nref, int mode);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,

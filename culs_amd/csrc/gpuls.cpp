@@ -126,7 +126,14 @@ int rewriteHandler(Shell &sh, const std::vector<std::string> &v) {
 
 int balanceHandler(Shell &sh, const std::vector<std::string> &v) {
     if (!flags_ok(v, "sv")) return 1;
-    printf("b: balancing is not available in this build\n");
+    if (!need_aig(sh, "balance")) return 0;
+    if (has_flag(v, "-s")) {
+        printf("b -s: only the level-then-literal input order is available; ignored\n");
+        return 0;
+    }
+    // balance, then strash to update levels (command_manager.cpp:77)
+    int rc = timed(sh, [&] { return gls_balance(sh.aig, 1, 1); });
+    if (rc != GLS_OK) report_error(sh, "balance", rc);
     return 0;
 }
 
@@ -143,6 +150,7 @@ struct CmdMan {
         registerCommand("rw", rewriteHandler);
         registerCommand("rwz", rewriteHandler);
         registerCommand("b", balanceHandler);
+        registerCommand("balance", balanceHandler);
     }
     int launch(Shell &sh, const std::vector<std::string> &v) {
         auto it = table.find(v[0]);

@@ -111,6 +111,15 @@ int gls_strash(gls_aig *aig, int32_t flags);
 #define GLS_RW_DATAFLOW_CUTS 4
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
+/* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing
+ * existing nodes.  Replaces AIGMan::balance (aig_manager.cu:682-737) and
+ * balancePerformV2 (src/algorithms/balance.cu:912-1382).  sort_dec_id must be
+ * 1 (inputs ordered by level, then literal; `b -s` is not supported).
+ * update_level = 1 strashes afterwards as the `b` command does
+ * (command_manager.cpp:77); with 0 the result keeps its balance numbering
+ * and its depth is unknown (levels = -1), as after `gb`. */
+int gls_balance(gls_aig *aig, int32_t sort_dec_id, int32_t update_level);
+
 /* Info about the last gls_strash / gls_rewrite on this handle. */
 int gls_aig_last_op(gls_aig *aig, gls_op_info *out);
 

@@ -7,6 +7,7 @@
  *
  * Reference files restated (paths relative to the CULS tree):
  *   strash -c         src/aig/strash.cu:322-466  (host strash of `st -c`)
+ *   balance           src/algorithms/balance.cu:666-1382 (`b`, sortDecId = 1)
  *   strash            src/aig/strash.cu:175-314  (GPU strash: dangling removal,
  *                     level waves, trivial folding, re-id by tentative id),
  *                     src/hash_table/hash_table.h:422-449 (key, trivial rules)
@@ -305,6 +306,308 @@ int orc_strash_cpu(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int 
     map_free(&m); free(newId); free(live);
     if (pDepth) *pDepth = depth;
     return next;
+}
+
+/* ------------------------------------------------------------------ */
+/* Balance `b` (balancePerformV2, src/algorithms/balance.cu:912-1382)   */
+/* ------------------------------------------------------------------ */
+/* Literals are AIGER here (0 = FALSE); where the reference's result     */
+/* depends on literal VALUES (input order ties) its own values are used, */
+/* in which lit 0 is TRUE (common.h:285-286): culs_val().                */
+#define BAL_STACK 128            /* DFS_COVER_STACK_SIZE, balance.cu:18 */
+#define BAL_MAX_COVER 1024       /* MAX_LOCAL_COVER_SIZE, balance.cu:17 */
+#define BAL_BATCH 32768          /* iterLen, balance.cu:1129 */
+
+typedef struct { int32_t *lit; int len; } BCover;   /* len -1: not computed */
+
+/* AND with the trivial rules of checkTrivialAndCases (hash_table.h:435-449);
+ * returns the literal, or -1 if non-trivial */
+static int64_t bal_trivial(uint32_t a, uint32_t b) {
+    if (a == b) return a;
+    if (a == (b ^ 1u)) return 0;                 /* FALSE */
+    if (a == 1u) return b;                       /* TRUE & b */
+    if (b == 1u) return a;
+    if (a == 0u || b == 0u) return 0;
+    return -1;
+}
+
+/* localCoverTravToTable (balance.cu:666-750): DFS from the root, fanin0 first;
+ * a literal is an input if complemented, not an AND, multi-fanout or at a
+ * limit; a repeated input is dropped; an input and its complement make the
+ * cover constant FALSE (len 0). */
+static void bal_cover(int root, int nPIs, const uint32_t *f0, const uint32_t *f1, const int32_t *nFo, BCover *c) {
+    uint32_t stack[BAL_STACK];
+    int top = -1;
+    int32_t *in = malloc(sizeof(int32_t) * (BAL_MAX_COVER + 64));   /* the limit is checked before expanding */
+    int len = 0;
+    const uint32_t rootLit = 2u * (uint32_t)root;
+    stack[++top] = rootLit;
+    while (top != -1) {
+        uint32_t cur = stack[top--];
+        int rep = 0;
+        for (int i = 0; i < len; i++) {
+            if ((uint32_t)in[i] == cur) { rep = 1; break; }
+            if ((uint32_t)in[i] == (cur ^ 1u)) { c->lit = in; c->len = 0; return; }
+        }
+        if (rep) continue;
+        int id = (int)ID(cur);
+        if (cur != rootLit &&
+            (CMP(cur) || id <= nPIs || nFo[id] > 1 || top + 2 >= BAL_STACK ||
+             len + 2 * (top + 1) + 1 >= BAL_MAX_COVER)) {
+            in[len++] = (int32_t)cur;
+            continue;
+        }
+        stack[++top] = f1[id];
+        stack[++top] = f0[id];
+    }
+    c->lit = in;
+    c->len = len;
+}
+
+/* level of a literal in the rebuilt graph (PIs and constants 0) */
+static uint32_t bal_level(const Map *lev, uint32_t lit, int nPIs) {
+    int id = (int)ID(lit);
+    if (id <= nPIs) return 0;
+    int32_t l = map_get(lev, (uint64_t)id);
+    return l < 0 ? 0u : (uint32_t)l;
+}
+
+/* AND lookup of retrieveAndNode (balance.cu:87-111): trivial result, or the
+ * existing node's literal, or -1 */
+static int64_t bal_lookup(const Map *tab, uint32_t a, uint32_t b) {
+    int64_t t = bal_trivial(a, b);
+    if (t >= 0) return t;
+    if (a > b) { uint32_t x = a; a = b; b = x; }
+    int32_t v = map_get(tab, and_key(a, b));
+    return v < 0 ? -1 : 2 * (int64_t)v;
+}
+
+int orc_balance(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos) {
+    int32_t *nFo = calloc((size_t)n, sizeof(int32_t));
+    for (int v = nPIs + 1; v < n; v++) { nFo[ID(f0[v])]++; nFo[ID(f1[v])]++; }
+    for (int i = 0; i < nPOs; i++) nFo[ID(pos[i])]++;
+    BCover *cov = malloc(sizeof(BCover) * (size_t)n);
+    for (int v = 0; v < n; v++) { cov[v].lit = NULL; cov[v].len = -1; }
+    int32_t *lastLevel = malloc(sizeof(int32_t) * (size_t)n);
+    for (int v = 0; v < n; v++) lastLevel[v] = -1;
+    /* phase 1: cover roots level by level from the POs (balance.cu:958-1037) */
+    int32_t *list = malloc(sizeof(int32_t) * (size_t)(n + nPOs + 1)), *next = malloc(sizeof(int32_t) * (size_t)(n + 1));
+    uint8_t *mark = calloc((size_t)n, 1);
+    int len = 0;
+    for (int i = 0; i < nPOs; i++) {
+        int id = (int)ID(pos[i]);
+        if (id > nPIs && !mark[id]) { mark[id] = 1; list[len++] = id; }
+    }
+    int levels = 0;
+    memset(mark, 0, (size_t)n);
+    while (len > 0) {
+        int nlen = 0;
+        for (int k = 0; k < len; k++) {
+            int v = list[k];
+            lastLevel[v] = levels;
+            if (cov[v].len < 0) bal_cover(v, nPIs, f0, f1, nFo, &cov[v]);
+            for (int i = 0; i < cov[v].len; i++) {
+                int id = (int)ID((uint32_t)cov[v].lit[i]);
+                if (id > nPIs && !mark[id]) { mark[id] = 1; next[nlen++] = id; }
+            }
+        }
+        for (int k = 0; k < nlen; k++) mark[next[k]] = 0;
+        memcpy(list, next, sizeof(int32_t) * (size_t)nlen);   /* order within a level does not matter */
+        len = nlen;
+        levels++;
+    }
+    /* covers by (last level, id) */
+    int nCov = 0;
+    for (int v = nPIs + 1; v < n; v++) if (lastLevel[v] >= 0) nCov++;
+    int32_t *byLevel = malloc(sizeof(int32_t) * (size_t)(nCov + 1));
+    int32_t *lvStart = calloc((size_t)levels + 2, sizeof(int32_t));
+    for (int v = nPIs + 1; v < n; v++) if (lastLevel[v] >= 0) lvStart[lastLevel[v] + 1]++;
+    for (int l = 1; l <= levels + 1; l++) lvStart[l] += lvStart[l - 1];
+    {
+        int32_t *fill = malloc(sizeof(int32_t) * (size_t)(levels + 1));
+        memcpy(fill, lvStart, sizeof(int32_t) * (size_t)(levels + 1));
+        for (int v = nPIs + 1; v < n; v++) if (lastLevel[v] >= 0) byLevel[fill[lastLevel[v]]++] = v;
+        free(fill);
+    }
+    /* phase 2: rebuild from the deepest level (balance.cu:1135-1229) */
+    Map tab; map_init(&tab, 2 * (int64_t)n + 16);     /* (lit0, lit1) -> id */
+    Map lev; map_init(&lev, 2 * (int64_t)n + 16);     /* id -> level */
+    uint32_t *newLit = malloc(sizeof(uint32_t) * (size_t)n);
+    int64_t counter = nPIs + 1;
+    int32_t *arr = malloc(sizeof(int32_t) * (size_t)BAL_BATCH * (BAL_MAX_COVER + 64));
+    uint32_t *alv = malloc(sizeof(uint32_t) * (size_t)BAL_BATCH * (BAL_MAX_COVER + 64));
+    int32_t *alen = malloc(sizeof(int32_t) * BAL_BATCH);
+    int8_t *mask = malloc(BAL_BATCH);
+    uint32_t *sval = malloc(sizeof(uint32_t) * BAL_BATCH), *slev = malloc(sizeof(uint32_t) * BAL_BATCH);
+    uint64_t *skey = malloc(sizeof(uint64_t) * BAL_BATCH);
+    int64_t nCreated = 0;
+    for (int l = levels - 1; l >= 0; l--) {
+        for (int b0 = lvStart[l]; b0 < lvStart[l + 1]; b0 += BAL_BATCH) {
+            int nb = lvStart[l + 1] - b0 < BAL_BATCH ? lvStart[l + 1] - b0 : BAL_BATCH;
+            /* gather and sort (prepareReconstructArrays, balance.cu:272-427) */
+            for (int c = 0; c < nb; c++) {
+                int v = byLevel[b0 + c];
+                int32_t *A = arr + (size_t)c * (BAL_MAX_COVER + 64);
+                uint32_t *LV = alv + (size_t)c * (BAL_MAX_COVER + 64);
+                alen[c] = cov[v].len;
+                for (int i = 0; i < cov[v].len; i++) {
+                    uint32_t lit = (uint32_t)cov[v].lit[i];
+                    int id = (int)ID(lit);
+                    uint32_t nl = id <= nPIs ? lit : (newLit[id] ^ CMP(lit));
+                    A[i] = (int32_t)nl;
+                    LV[i] = bal_level(&lev, nl, nPIs);
+                }
+                /* level descending, then the reference's literal value descending (insertion sort) */
+                for (int i = 1; i < alen[c]; i++) {
+                    int32_t x = A[i]; uint32_t xl = LV[i];
+                    int j = i - 1;
+                    while (j >= 0 && (LV[j] < xl || (LV[j] == xl && culs_val((uint32_t)A[j]) < culs_val((uint32_t)x)))) {
+                        A[j + 1] = A[j]; LV[j + 1] = LV[j]; j--;
+                    }
+                    A[j + 1] = x; LV[j + 1] = xl;
+                }
+            }
+            for (;;) {
+                /* sharedNodeDrivenPermute (balance.cu:187-247), against the table as it stands */
+                for (int c = 0; c < nb; c++) {
+                    int L = alen[c];
+                    if (L <= 1) continue;
+                    int32_t *A = arr + (size_t)c * (BAL_MAX_COVER + 64);
+                    uint32_t *LV = alv + (size_t)c * (BAL_MAX_COVER + 64);
+                    int left;
+                    if (L < 3) left = 0;
+                    else {
+                        left = L - 2;
+                        uint32_t border = LV[left];
+                        for (left--; left >= 0; left--) if (LV[left] != border) break;
+                        left++;
+                    }
+                    int right = L - 2;
+                    if (left < right) {
+                        uint32_t l1 = (uint32_t)A[right + 1], l2 = (uint32_t)A[right];
+                        for (int i = right; i >= left; i--) {
+                            uint32_t l3 = (uint32_t)A[i];
+                            if (bal_lookup(&tab, l1, l3) >= 0) {
+                                if (l3 != l2) {
+                                    A[i] = (int32_t)l2; A[right] = (int32_t)l3;
+                                    uint32_t t = LV[i]; LV[i] = LV[right]; LV[right] = t;
+                                }
+                                break;
+                            }
+                        }
+                    }
+                }
+                /* prepareDataToInsert (balance.cu:113-185) */
+                int any = 0;
+                for (int c = 0; c < nb; c++) {
+                    int L = alen[c];
+                    int32_t *A = arr + (size_t)c * (BAL_MAX_COVER + 64);
+                    uint32_t *LV = alv + (size_t)c * (BAL_MAX_COVER + 64);
+                    if (L <= 1) { mask[c] = 0; continue; }
+                    uint32_t l1 = (uint32_t)A[L - 1], l2 = (uint32_t)A[L - 2];
+                    uint32_t v1 = LV[L - 1], v2 = LV[L - 2];
+                    alen[c] = L - 2;
+                    int64_t r = bal_lookup(&tab, l1, l2);
+                    any = 1;
+                    if (r >= 0) {
+                        mask[c] = 2;
+                        sval[c] = (uint32_t)r;
+                        slev[c] = bal_level(&lev, (uint32_t)r, nPIs);
+                        continue;
+                    }
+                    if (l1 > l2) { uint32_t t = l1; l1 = l2; l2 = t; }
+                    mask[c] = 1;
+                    skey[c] = and_key(l1, l2);
+                    sval[c] = (uint32_t)(counter + c);
+                    slev[c] = (v1 > v2 ? v1 : v2) + 1u;
+                }
+                if (!any) break;
+                /* insert with the smallest id per key, then read back (balance.cu:1188-1210) */
+                int64_t maxId = 0;
+                for (int c = 0; c < nb; c++) {
+                    if (mask[c] != 1) continue;
+                    int32_t got = map_put(&tab, skey[c], (int32_t)sval[c]);   /* smaller c inserted first */
+                    if (got == (int32_t)sval[c]) nCreated++;
+                    sval[c] = (uint32_t)got;
+                }
+                for (int c = 0; c < nb; c++) if (mask[c] == 1 && (int64_t)sval[c] > maxId) maxId = sval[c];
+                if (maxId != 0) counter = maxId + 1;
+                for (int c = 0; c < nb; c++) if (mask[c] == 1) map_put(&lev, (uint64_t)sval[c], (int32_t)slev[c]);
+                /* addBackLocalArrays (balance.cu:429-467) */
+                for (int c = 0; c < nb; c++) {
+                    if (mask[c] == 0) continue;
+                    uint32_t lit = mask[c] == 2 ? sval[c] : 2u * sval[c];
+                    int32_t *A = arr + (size_t)c * (BAL_MAX_COVER + 64);
+                    uint32_t *LV = alv + (size_t)c * (BAL_MAX_COVER + 64);
+                    int L = alen[c], dup = 0;
+                    for (int i = 0; i < L; i++) if ((uint32_t)A[i] == lit) { dup = 1; break; }
+                    if (dup) continue;
+                    A[L] = (int32_t)lit; LV[L] = slev[c];
+                    alen[c] = L + 1;
+                    for (int i = L; i > 0; i--) {
+                        if (LV[i] <= LV[i - 1]) break;
+                        int32_t t = A[i]; A[i] = A[i - 1]; A[i - 1] = t;
+                        uint32_t u = LV[i]; LV[i] = LV[i - 1]; LV[i - 1] = u;
+                    }
+                }
+            }
+            /* recordReconstructedCovers (balance.cu:469-491) */
+            for (int c = 0; c < nb; c++) {
+                int v = byLevel[b0 + c];
+                newLit[v] = alen[c] == 0 ? 0u : (uint32_t)arr[(size_t)c * (BAL_MAX_COVER + 64)];
+            }
+        }
+    }
+    int ret;
+    if (nCreated + nPIs + 1 > cap) ret = -2;
+    else {
+        /* re-id: created nodes in id order (balance.cu:1237-1262) */
+        int64_t m = 0;
+        uint64_t *keys = malloc(sizeof(uint64_t) * (size_t)(nCreated + 1));
+        uint32_t *ids = malloc(sizeof(uint32_t) * (size_t)(nCreated + 1));
+        for (uint64_t s = 0; s <= tab.mask; s++)
+            if (tab.k[s] != MAP_EMPTY) { keys[m] = tab.k[s]; ids[m] = (uint32_t)tab.v[s]; m++; }
+        /* sort by id (insertion into an id-indexed order via counting on a copy) */
+        int64_t *ord = malloc(sizeof(int64_t) * (size_t)(m + 1));
+        for (int64_t i = 0; i < m; i++) ord[i] = i;
+        /* shell sort on ids: m is the number of created nodes */
+        for (int64_t gap = m / 2; gap > 0; gap /= 2)
+            for (int64_t i = gap; i < m; i++) {
+                int64_t t = ord[i], j = i;
+                while (j >= gap && ids[ord[j - gap]] > ids[t]) { ord[j] = ord[j - gap]; j -= gap; }
+                ord[j] = t;
+            }
+        Map outId; map_init(&outId, m + 16);
+        for (int64_t i = 0; i < m; i++) map_put(&outId, (uint64_t)ids[ord[i]], (int32_t)(nPIs + 1 + i));
+        for (int v = 0; v <= nPIs; v++) { of0[v] = 0; of1[v] = 0; }
+        for (int64_t i = 0; i < m; i++) {
+            uint64_t k = keys[ord[i]];
+            uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+            uint32_t ia = ID(a), ib = ID(b);
+            uint32_t na = ia <= (uint32_t)nPIs ? ia : (uint32_t)map_get(&outId, ia);
+            uint32_t nb2 = ib <= (uint32_t)nPIs ? ib : (uint32_t)map_get(&outId, ib);
+            of0[nPIs + 1 + i] = 2u * na | CMP(a);
+            of1[nPIs + 1 + i] = 2u * nb2 | CMP(b);
+        }
+        /* POs (processPO, balance.cu:882-910) */
+        for (int i = 0; i < nPOs; i++) {
+            uint32_t l = pos[i];
+            int id = (int)ID(l);
+            if (id <= nPIs) { opos[i] = l; continue; }
+            uint32_t nl = newLit[id];
+            uint32_t nid = ID(nl);
+            uint32_t oid = nid <= (uint32_t)nPIs ? nid : (uint32_t)map_get(&outId, nid);
+            opos[i] = 2u * oid | (CMP(l) ^ CMP(nl));
+        }
+        ret = (int)(nPIs + 1 + m);
+        map_free(&outId); free(ord); free(keys); free(ids);
+    }
+    for (int v = 0; v < n; v++) free(cov[v].lit);
+    free(cov); free(nFo); free(lastLevel); free(list); free(next); free(mark); free(byLevel); free(lvStart);
+    map_free(&tab); map_free(&lev); free(newLit); free(arr); free(alv); free(alen); free(mask);
+    free(sval); free(slev); free(skey);
+    return ret;
 }
 
 /* ------------------------------------------------------------------ */

@@ -33,6 +33,12 @@ int orc_strash_cpu(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, 
                    const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos, uint32_t *olevel,
                    int *pDepth);
 
+/* Balance `b` (sortDecId = 1) without the strash that the `b` command runs
+ * afterwards.  cap: capacity of of0/of1.  Returns nObjs of the result, or
+ * -2 if it would exceed cap. */
+int orc_balance(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos);
+
 /* Level-major renumbering (ties by id) of an AIG whose ids are topological. */
 int orc_reorder(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,
                 const uint32_t *pos, uint32_t *of0, uint32_t *of1, uint32_t *opos,

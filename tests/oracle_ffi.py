@@ -81,6 +81,19 @@ def strash_cpu(nPIs, f0, f1, pos):
     return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy()), ol[:r].copy(), d.value
 
 
+def balance(nPIs, f0, f1, pos):
+    """`b` without its closing strash (sortDecId = 1)."""
+    f0, f1, pos = u32(f0), u32(f1), u32(pos)
+    n = len(f0)
+    cap = 3 * n + 64
+    o0, o1 = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+    op = np.zeros(max(len(pos), 1), np.uint32)
+    r = lib().orc_balance(n, nPIs, P(f0), P(f1), len(pos), P(pos), cap, P(o0), P(o1), P(op))
+    if r < 0:
+        raise RuntimeError("orc_balance failed %d" % r)
+    return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy())
+
+
 def reorder(nPIs, f0, f1, pos):
     f0, f1, pos = u32(f0), u32(f1), u32(pos)
     n = len(f0)
