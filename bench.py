@@ -59,8 +59,12 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
                     help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
-    ap.add_argument("--workload", choices=["rw10m", "batch"], default="rw10m",
-                    help="rw10m: the headline line; batch: 4096 mixed AIGs packed into one arena vs one at a time")
+    ap.add_argument("--workload", choices=["rw10m", "batch", "serve"], default="rw10m",
+                    help="rw10m: the headline line; batch: 4096 mixed AIGs packed into one arena vs one at a "
+                         "time; serve: a Poisson stream of `b; rw; rwz; b` jobs, served vs a per-command loop")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second (Poisson)")
+    ap.add_argument("--serve-max", type=int, default=2_000_000, help="largest AND count of a random job")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-max", type=int, default=200_000, help="largest AND count of the batch mix")
     ap.add_argument("--batch-check", type=int, default=256,
@@ -85,6 +89,8 @@ def main():
 
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "serve":
+        return bench_serve(a)
     ctx = culs_amd.Context(0)
     if not ctx.has_device:
         raise SystemExit("no GPU")
@@ -268,6 +274,106 @@ def bench_batch(a):
         "qor": {"and_in": n_in, "and_out": n_out},
         "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
                                                       "build_ms", "strash_ms", "total_ms")},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_serve(a):
+    """Serving: --serve-requests jobs arriving as a Poisson stream (--serve-rate
+    per second), each `b; rw; rwz; b`; half seeded random AIGs with ANDs
+    log-uniform in [1000, --serve-max] (PIs = max(16, ANDs/200), POs = PIs/2,
+    fanin window 1000), half n-bit array multipliers, n uniform in 8..64.
+    Served: the continuous-batching server (jobs stay on the device for their
+    whole script).  Baseline: one job at a time in arrival order, the AIG
+    copied to the host and back around every command, as the reference does
+    around rw.  Latency = completion - arrival."""
+    import threading
+    import numpy as np
+    import culs_amd
+    script = "b; rw; rwz; b"
+    rng = np.random.default_rng(777)
+    ctx = culs_amd.Context(0)
+    t0 = time.time()
+    jobs, arrivals, t = [], [], 0.0
+    for i in range(a.serve_requests):
+        t += rng.exponential(1.0 / a.serve_rate)
+        arrivals.append(t)
+        if rng.random() < 0.5:
+            n = int(np.exp(rng.uniform(np.log(1000), np.log(a.serve_max))))
+            pis = max(16, n // 200)
+            jobs.append(ctx.gen_random(pis, n, max(1, pis // 2), 1000, 2000 + i).arrays())
+        else:
+            jobs.append(ctx.gen_multiplier(int(rng.integers(8, 65)), 0.0, 2000 + i).arrays())
+    n_in = sum(len(g[1]) - g[0] - 1 for g in jobs)
+    log("generated %d jobs (%d ANDs) in %.1fs" % (len(jobs), n_in, time.time() - t0))
+
+    def pct(x, q):
+        return float(np.percentile(np.asarray(x), q))
+
+    # served
+    srv = culs_amd.Server(0, hbm_budget=96 << 30, max_active=4096)
+    warm = srv.submit(jobs[0], script)
+    srv.wait(warm)
+    srv.release(warm)
+    ids = [None] * len(jobs)
+    start = time.perf_counter()
+
+    def submitter():
+        for k, g in enumerate(jobs):
+            d = arrivals[k] - (time.perf_counter() - start)
+            if d > 0:
+                time.sleep(d)
+            ids[k] = srv.submit(g, script)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    infos = [srv.wait(j) for j in ids]
+    served_span = time.perf_counter() - start
+    lat_s = [(i["done_ms"] - i["submit_ms"]) for i in infos]
+    served_out = [srv.result(j) for j in ids]
+    srv.close()
+    log("served: makespan %.2fs p50 %.1f ms p99 %.1f ms" % (served_span, pct(lat_s, 50), pct(lat_s, 99)))
+
+    # plain loop with a host round trip per command
+    lat_l, loop_out = [], []
+    start = time.perf_counter()
+    for k, g in enumerate(jobs):
+        d = arrivals[k] - (time.perf_counter() - start)
+        if d > 0:
+            time.sleep(d)
+        cur = g
+        for cmd in ("b", "rw", "rwz", "b"):
+            x = ctx.from_arrays(*cur)
+            if cmd == "b":
+                x.balance()
+            else:
+                x.rewrite(use_zeros=(cmd == "rwz"))
+            cur = x.arrays()
+            x.free()
+        loop_out.append(cur)
+        lat_l.append((time.perf_counter() - start - arrivals[k]) * 1e3)
+    loop_span = time.perf_counter() - start
+    log("loop: makespan %.2fs p50 %.1f ms p99 %.1f ms" % (loop_span, pct(lat_l, 50), pct(lat_l, 99)))
+    identical = all(all(np.array_equal(p, q) for p, q in zip(x, y)) for x, y in zip(served_out, loop_out))
+    res = {
+        "metric": "served b;rw;rwz;b Mnodes/s, Poisson %g req/s" % a.serve_rate,
+        "value": round(n_in / served_span / 1e6, 4),
+        "unit": "Mnodes/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(served_span * 1e3, 3),
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "%d jobs `%s`, Poisson arrivals %g/s; half random AIGs (ANDs log-uniform "
+                               "[1000, %d], window 1000), half n-bit multipliers (n in 8..64); seeds 2000+i"
+                               % (len(jobs), script, a.serve_rate, a.serve_max),
+                   "ands_total": n_in},
+        "served": {"p50_ms": round(pct(lat_s, 50), 3), "p99_ms": round(pct(lat_s, 99), 3),
+                   "makespan_s": round(served_span, 3), "jobs_per_s": round(len(jobs) / served_span, 3)},
+        "loop": {"p50_ms": round(pct(lat_l, 50), 3), "p99_ms": round(pct(lat_l, 99), 3),
+                 "makespan_s": round(loop_span, 3), "jobs_per_s": round(len(jobs) / loop_span, 3),
+                 "value": round(n_in / loop_span / 1e6, 4)},
+        "throughput_vs_loop": round(loop_span / served_span, 3),
+        "identical_to_loop": bool(identical),
     }
     print(json.dumps(res), flush=True)
 

@@ -24,6 +24,16 @@ class GlsError(RuntimeError):
     pass
 
 
+class JobInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("nPIs", ctypes.c_int32), ("nPOs", ctypes.c_int32),
+                ("nAnds", ctypes.c_int32), ("levels", ctypes.c_int32), ("nAnds_in", ctypes.c_int32),
+                ("footprint_bytes", ctypes.c_int64), ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("nPIs", ctypes.c_int32), ("nPOs", ctypes.c_int32), ("nAnds", ctypes.c_int32),
                 ("nObjs", ctypes.c_int32), ("levels", ctypes.c_int32), ("on_device", ctypes.c_int32)]
@@ -63,8 +73,17 @@ _SIGS = {
     "gls_rewrite": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "gls_aig_last_op": (ctypes.c_int, [_P, ctypes.POINTER(OpInfo)]),
     "gls_balance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "gls_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "gls_server_submit": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
+                                         ctypes.c_int32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "gls_server_wait": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(JobInfo)]),
+    "gls_server_result": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_int32]),
+    "gls_server_release": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "gls_server_last_error": (ctypes.c_char_p, [_P]),
+    "gls_server_destroy": (None, [_P]),
     "gls_batch_strash": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_rewrite": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "gls_batch_balance": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "gls_reorder": (ctypes.c_int, [_P, _P]),
     "gls_enumerate_cuts": (ctypes.c_int, [_P, _P]),
     "gls_evaluate": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
@@ -177,6 +196,12 @@ class Context:
                    "gls_batch_rewrite")
         return aigs
 
+    def batch_balance(self, aigs, update_level=True):
+        """`b` over every AIG of the list in one device pass (results as if alone)."""
+        self.check(lib().gls_batch_balance(self._h, self._handles(aigs), len(aigs), 1, 1 if update_level else 0),
+                   "gls_batch_balance")
+        return aigs
+
     def gen_multiplier(self, nbits, redundant, seed):
         h = _P()
         self.check(lib().gls_gen_multiplier(self._h, nbits, redundant, seed, ctypes.byref(h)), "gen_multiplier")
@@ -282,3 +307,59 @@ def npn_tables():
     if rc != 0:
         raise GlsError("gls_npn_tables failed")
     return ph, pe, cl, p4.reshape(24, 4)
+
+
+class Server:
+    """Continuous-batching rewrite server (gls_server_*): submit(arrays, script)
+    returns a job id; wait(job) -> info dict; result(job) -> arrays."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, max_active=4096):
+        self._h = _P()
+        rc = lib().gls_server_create(device, hbm_budget, max_active, ctypes.byref(self._h))
+        if rc != 0:
+            raise GlsError("gls_server_create failed (%d)" % rc)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().gls_server_last_error(self._h)
+            raise GlsError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def submit(self, g, script):
+        nPIs, f0, f1, pos = g
+        f0 = np.ascontiguousarray(f0, dtype=np.uint32)
+        f1 = np.ascontiguousarray(f1, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos, dtype=np.uint32)
+        job = ctypes.c_int64(0)
+        self._check(lib().gls_server_submit(self._h, nPIs, len(pos), len(f0) - nPIs - 1, _ptr(f0), _ptr(f1),
+                                            _ptr(pos) if len(pos) else None, GLS_LIT_AIGER, script.encode(),
+                                            ctypes.byref(job)), "gls_server_submit")
+        return job.value
+
+    def wait(self, job, timeout_ms=-1.0):
+        info = JobInfo()
+        self._check(lib().gls_server_wait(self._h, job, timeout_ms, ctypes.byref(info)), "gls_server_wait")
+        return info.as_dict()
+
+    def result(self, job):
+        info = self.wait(job)
+        n = info["nPIs"] + 1 + info["nAnds"]
+        f0 = np.zeros(n, np.uint32)
+        f1 = np.zeros(n, np.uint32)
+        pos = np.zeros(max(1, info["nPOs"]), np.uint32)
+        self._check(lib().gls_server_result(self._h, job, _ptr(f0), _ptr(f1), _ptr(pos), GLS_LIT_AIGER),
+                    "gls_server_result")
+        return info["nPIs"], f0, f1, pos[:info["nPOs"]]
+
+    def release(self, job):
+        self._check(lib().gls_server_release(self._h, job), "gls_server_release")
+
+    def close(self):
+        if self._h:
+            lib().gls_server_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

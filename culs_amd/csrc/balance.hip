@@ -265,7 +265,13 @@ struct RecArgs {
     uint32_t *sval, *slev;
     uint32_t *counter;                // [0] next tentative id, [1] step max, [2] any, [3] created
     uint32_t *err;                    // 2: table full
+    const uint32_t *tagIn;            // optional: per-node tag of the input (batched balance)
+    Tables tagTab;                    // created id -> tag of the cover that made it
 };
+
+__device__ __forceinline__ void rec_tag(const RecArgs &r, uint32_t c, uint32_t root) {
+    if (r.tagIn && r.mask[c] == 1 && !l_insert(r.tagTab, r.sval[c], r.tagIn[root])) atomicOr(r.err, 2u);
+}
 
 // prepareReconstructArrays (balance.cu:272-427), one cover
 __device__ void rec_gather(const RecArgs &r, uint32_t c, uint32_t root) {
@@ -405,6 +411,7 @@ __global__ void __launch_bounds__(BAL_WG) k_rec_narrow(RecArgs r, const uint32_t
             if (tid == 0 && s_max) s_counter = s_max + 1u;
             for (uint32_t c = tid; c < nb; c += BAL_WG) {
                 if (r.mask[c] == 1 && !l_insert(r.t, r.sval[c], r.slev[c])) atomicOr(r.err, 2u);
+                rec_tag(r, c, r.roots[s + c]);
                 rec_add_back(r, c);
             }
             __syncthreads();
@@ -450,10 +457,11 @@ __global__ void k_rec_retrieve(RecArgs r, uint32_t nb) {
     if (mx) atomicMax(&r.counter[1], mx);
 }
 
-__global__ void k_rec_add_back(RecArgs r, uint32_t nb) {
+__global__ void k_rec_add_back(RecArgs r, uint32_t s, uint32_t nb) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nb; c += stride) {
         if (r.mask[c] == 1 && !l_insert(r.t, r.sval[c], r.slev[c])) atomicOr(r.err, 2u);
+        rec_tag(r, (uint32_t)c, r.roots[s + c]);
         rec_add_back(r, (uint32_t)c);
     }
 }
@@ -498,8 +506,9 @@ __device__ __forceinline__ uint32_t out_lit(const Tables &outMap, uint32_t lit, 
     return 2u * l_find(outMap, id) | (lit & 1u);
 }
 
-__global__ void k_out_emit(int32_t m, int32_t nPIs, const unsigned long long *__restrict__ sortedKeys, Tables outMap,
-                           uint32_t *of0, uint32_t *of1, uint32_t *olevel) {
+__global__ void k_out_emit(int32_t m, int32_t nPIs, const unsigned long long *__restrict__ sortedKeys,
+                           const uint32_t *__restrict__ sortedIds, Tables outMap, uint32_t *of0, uint32_t *of1,
+                           uint32_t *olevel, const uint32_t *__restrict__ tagIn, Tables tagTab, uint32_t *tagOut) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < m; q += stride) {
         const unsigned long long k = sortedKeys[q];
@@ -507,11 +516,13 @@ __global__ void k_out_emit(int32_t m, int32_t nPIs, const unsigned long long *__
         of0[o] = out_lit(outMap, (uint32_t)(k >> 32), nPIs);
         of1[o] = out_lit(outMap, (uint32_t)k, nPIs);
         olevel[o] = 0;
+        if (tagOut) tagOut[o] = l_find(tagTab, sortedIds[q]);
     }
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v <= nPIs; v += stride) {
         of0[v] = 0;
         of1[v] = 0;
         olevel[v] = 0;
+        if (tagOut) tagOut[v] = tagIn[v];
     }
 }
 
@@ -549,7 +560,9 @@ uint32_t bits_of(uint32_t x) {
 
 } // namespace
 
-int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info) {
+int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn,
+                uint32_t **tagOut) {
+    const bool tagged = tagIn && tagOut;
     const int32_t n = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs;
     hipStream_t st = ctx->stream;
     Tmp t{ctx};
@@ -619,6 +632,25 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
         for (uint32_t i = 0; i < nRoots; i++) starts[hk[i] + 1]++;
         for (uint32_t l = 1; l <= levels + 1; l++) starts[l] += starts[l - 1];
     }
+    // Batches: 32768 covers of a level at a time.  With tags (a union of
+    // independent AIGs) a level is one batch, which gives every AIG the batches
+    // it gets alone as long as none has more than 32768 covers on a level.
+    if (tagged && nRoots > 0) {
+        std::vector<uint32_t> hk(nRoots), hr(nRoots), htag(n);
+        GLS_TRY(stream_copy(ctx, hk.data(), keysSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
+        GLS_TRY(stream_copy(ctx, hr.data(), rootsSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
+        GLS_TRY(stream_copy(ctx, htag.data(), tagIn, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        std::vector<std::pair<uint64_t, uint32_t>> groups;
+        groups.reserve(nRoots);
+        for (uint32_t i = 0; i < nRoots; i++) groups.push_back({(uint64_t)hk[i] << 32 | htag[hr[i]], 0u});
+        std::sort(groups.begin(), groups.end());
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < nRoots; i++) {
+            run = (i > 0 && groups[i].first == groups[i - 1].first) ? run + 1 : 1;
+            if (run > (uint32_t)BAL_BATCH) return GLS_ERR_BATCH_SPLIT;
+        }
+    }
+    const uint32_t batchLimit = tagged ? 0xFFFFFFFFu : (uint32_t)BAL_BATCH;
     // maximum cover length
     uint32_t maxLen = 1;
     {
@@ -635,7 +667,8 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     uint32_t *rv = t.get<uint32_t>(cap), *lk = t.get<uint32_t>(cap), *lv = t.get<uint32_t>(cap);
     uint32_t *newLit = t.get<uint32_t>(n);
     size_t maxBatch = 1;
-    for (uint32_t l = 0; l < levels; l++) maxBatch = std::max<size_t>(maxBatch, std::min<uint32_t>(starts[l + 1] - starts[l], BAL_BATCH));
+    for (uint32_t l = 0; l < levels; l++)
+        maxBatch = std::max<size_t>(maxBatch, std::min<uint32_t>(starts[l + 1] - starts[l], batchLimit));
     uint32_t *arr = t.get<uint32_t>(maxBatch * maxLen), *alv = t.get<uint32_t>(maxBatch * maxLen);
     uint32_t *alen = t.get<uint32_t>(maxBatch);
     uint8_t *mask = t.get<uint8_t>(maxBatch);
@@ -652,8 +685,16 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     const uint32_t firstId = (uint32_t)nPIs + 1u;
     GLS_TRY(stream_copy(ctx, &ctr[4], &firstId, sizeof(uint32_t), hipMemcpyHostToDevice));
     Tables tabs{rk, rv, cap - 1, lk, lv, cap - 1};
+    Tables tagTab{nullptr, nullptr, 0, nullptr, nullptr, 0};
+    if (tagged) {
+        uint32_t *tk = t.get<uint32_t>(cap), *tv = t.get<uint32_t>(cap);
+        if (!tk || !tv) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: tag table");
+        GLS_HIP(ctx, hipMemsetAsync(tk, 0xFF, sizeof(uint32_t) * cap, st));
+        GLS_HIP(ctx, hipMemsetAsync(tv, 0xFF, sizeof(uint32_t) * cap, st));
+        tagTab = Tables{nullptr, nullptr, 0, tk, tv, cap - 1};
+    }
     RecArgs ra{tabs, nPIs, rootsSorted, coverOff, coverLen, pool, newLit, arr, alv, alen, maxLen, mask, skey,
-               sval, slev, &ctr[4], &ctr[2]};
+               sval, slev, &ctr[4], &ctr[2], tagged ? tagIn : nullptr, tagTab};
     int32_t l = (int32_t)levels - 1;
     while (l >= 0) {
         // a run of narrow levels goes to the persistent workgroup
@@ -666,8 +707,8 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
             continue;
         }
         // one wide level, in batches, one kernel per phase
-        for (uint32_t b0 = starts[l]; b0 < starts[l + 1]; b0 += BAL_BATCH) {
-            const uint32_t nb = std::min<uint32_t>(starts[l + 1] - b0, BAL_BATCH);
+        for (uint32_t b0 = starts[l]; b0 < starts[l + 1]; b0 += std::min<uint32_t>(starts[l + 1] - b0, batchLimit)) {
+            const uint32_t nb = std::min<uint32_t>(starts[l + 1] - b0, batchLimit);
             k_rec_gather<<<nblocks(nb), kB, 0, st>>>(ra, b0, nb);
             for (;;) {
                 k_rec_prepare<<<nblocks(nb), kB, 0, st>>>(ra, nb);
@@ -676,7 +717,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
                 if (!any) break;
                 k_rec_insert<<<nblocks(nb), kB, 0, st>>>(ra, nb);
                 k_rec_retrieve<<<nblocks(nb), kB, 0, st>>>(ra, nb);
-                k_rec_add_back<<<nblocks(nb), kB, 0, st>>>(ra, nb);
+                k_rec_add_back<<<nblocks(nb), kB, 0, st>>>(ra, b0, nb);
                 k_rec_step_end<<<1, 1, 0, st>>>(&ctr[4]);
                 GLS_HIP(ctx, hipGetLastError());
             }
@@ -713,8 +754,14 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     if (m > 0) k_out_map<<<nblocks(m), kB, 0, st>>>((int32_t)m, nPIs, cidsS, outMap, &ctr[2]);
     if ((int64_t)nPIs + 1 + m > 0x7FFFFFF0LL) return gls_fail(ctx, GLS_ERR_ARG, "balance: result too large");
     GLS_TRY(dev_graph_alloc(ctx, out, nPIs + 1 + (int32_t)m, nPIs, nPOs));
-    k_out_emit<<<nblocks(std::max<int64_t>(m, nPIs + 1)), kB, 0, st>>>((int32_t)m, nPIs, ckeysS, outMap, out.f0,
-                                                                     out.f1, out.level);
+    uint32_t *tags = nullptr;
+    if (tagged) {
+        tags = pool_get<uint32_t>(ctx, (size_t)nPIs + 1 + m);
+        if (!tags) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "balance: tags"); }
+        *tagOut = tags;
+    }
+    k_out_emit<<<nblocks(std::max<int64_t>(m, nPIs + 1)), kB, 0, st>>>((int32_t)m, nPIs, ckeysS, cidsS, outMap,
+                                                                     out.f0, out.f1, out.level, tagIn, tagTab, tags);
     if (nPOs > 0) k_out_pos<<<nblocks(nPOs), kB, 0, st>>>(nPOs, nPIs, in.pos, newLit, outMap, out.pos);
     GLS_HIP(ctx, hipGetLastError());
     GLS_TRY(stream_copy(ctx, &herr, &ctr[2], sizeof(uint32_t), hipMemcpyDeviceToHost));

@@ -181,7 +181,7 @@ struct PoolBlock {
     }
 };
 
-enum class Pass { Strash, Rewrite };
+enum class Pass { Strash, Rewrite, Balance };
 
 int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t useZeros, int32_t flags) {
     if (!ctx || (count > 0 && !aigs) || count < 0) return gls_fail(ctx, GLS_ERR_ARG, "batch: bad arguments");
@@ -268,9 +268,15 @@ int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t u
     if (!dep) return gls_fail(ctx, GLS_ERR_NOMEM, "batch: depth");
     GLS_HIP(ctx, hipMemsetAsync(dep, 0, sizeof(uint32_t) * count, st));
     gls_op_info info = {};
-    int rc = pass == Pass::Strash ? dev_strash(ctx, U, R, &depth, &info, utag, &rtag, dep)
-                                  : dev_rewrite(ctx, U, useZeros, flags, R, &depth, &info, utag, &rtag, dep);
+    int rc = pass == Pass::Strash    ? dev_strash(ctx, U, R, &depth, &info, utag, &rtag, dep)
+             : pass == Pass::Rewrite ? dev_rewrite(ctx, U, useZeros, flags, R, &depth, &info, utag, &rtag, dep)
+                                     : dev_balance(ctx, U, R, &info, utag, &rtag);
     if (rtag) pb.ptrs.push_back(rtag);
+    if (rc == GLS_ERR_BATCH_SPLIT) {
+        // an AIG has a level wider than one balance batch: the members go alone
+        for (int32_t k = 0; k < count; k++) GLS_TRY(gls_balance(aigs[k], 1, 0));
+        return GLS_OK;
+    }
     if (rc != GLS_OK) return rc;
     struct GraphGuard rGuard{ctx, R};
 
@@ -335,11 +341,11 @@ int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t u
         a->dev = outs[k];
         a->nObjs = outs[k].nObjs;
         a->dev_valid = true;
-        a->dev_level_valid = true;
+        a->dev_level_valid = pass != Pass::Balance;
         a->host_valid = false;
         a->host_level_valid = false;
-        a->levels = (int32_t)hdep[k];
-        a->strashed = true;
+        a->levels = pass != Pass::Balance ? (int32_t)hdep[k] : -1;
+        a->strashed = pass != Pass::Balance;
         a->last = info;
     }
     oGuard.keep = true;
@@ -356,4 +362,13 @@ extern "C" int gls_batch_strash(gls_ctx *ctx, gls_aig **aigs, int32_t count, int
 
 extern "C" int gls_batch_rewrite(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t use_zeros, int32_t flags) {
     return batch_pass(ctx, aigs, count, Pass::Rewrite, use_zeros ? 1 : 0, flags);
+}
+
+extern "C" int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_dec_id,
+                                 int32_t update_level) {
+    if (sort_dec_id != 1)
+        return gls_fail(ctx, GLS_ERR_ARG, "batch balance: only sort_dec_id = 1 (level, then literal) is supported");
+    GLS_TRY(batch_pass(ctx, aigs, count, Pass::Balance, 0, 0));
+    if (update_level) GLS_TRY(batch_pass(ctx, aigs, count, Pass::Strash, 0, 0));
+    return GLS_OK;
 }

@@ -122,7 +122,12 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
 int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
                 gls_op_info *info, const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr,
                 uint32_t *tagDepth = nullptr);
-int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info);
+// tagIn / tagOut as in dev_strash.  Returns GLS_ERR_BATCH_SPLIT (without a
+// message) when one tag has more than 32768 covers on a level: that tag would
+// be rebuilt in other batches than alone.
+#define GLS_ERR_BATCH_SPLIT (-100)
+int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn = nullptr,
+                uint32_t **tagOut = nullptr);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
                        uint32_t *d_cuts, int32_t *d_nref, int mode);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,

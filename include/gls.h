@@ -31,6 +31,7 @@ extern "C" {
 #define GLS_ERR_NODEV (-5)    /* no GPU: context is host-only */
 #define GLS_ERR_IO (-6)       /* file open/read/write failed */
 #define GLS_ERR_INTERNAL (-7) /* an internal consistency check failed */
+#define GLS_ERR_TIMEOUT (-8)  /* gls_server_wait: the job is not done yet */
 
 #define GLS_LIT_AIGER 0
 #define GLS_LIT_CULS 1
@@ -134,6 +135,44 @@ int gls_aig_last_op(gls_aig *aig, gls_op_info *out);
  * member then reports the batch as a whole. */
 int gls_batch_strash(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t flags);
 int gls_batch_rewrite(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t use_zeros, int32_t flags);
+/* Batched gls_balance: one device pass over the packed members; a member with
+ * more than 32768 covers on one level is balanced alone (same result). */
+int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_dec_id, int32_t update_level);
+
+/* ---- serving ---------------------------------------------------------- */
+/* No counterpart in the reference (one script, one AIG, one process).  A
+ * server owns a worker thread and its own context on `device`.  Jobs (an AIG
+ * as arrays + a script of st / strash, rw, rwz / rw -z, b / balance separated
+ * by ';') are admitted in arrival order while their estimated device
+ * footprint fits hbm_budget_bytes (and at most max_active run at once); an
+ * admitted AIG stays on the device for its whole script.  Each round runs
+ * every command kind once as a batched pass over the active jobs whose next
+ * command it is, so jobs that arrive while others run join the next round.
+ * A job's result equals running its script alone.  All functions are
+ * thread-safe; gls_server_last_error's text is valid until the next call. */
+typedef struct gls_server gls_server;
+typedef struct {
+    int32_t status;                 /* 0 queued, 1 on the device, 2 done, < 0 error code */
+    int32_t nPIs, nPOs, nAnds, levels;  /* the result */
+    int32_t nAnds_in;
+    int64_t footprint_bytes;        /* device bytes reserved while active */
+    double submit_ms, admit_ms, done_ms;  /* milliseconds since the server started */
+} gls_job_info;
+int gls_server_create(int device, int64_t hbm_budget_bytes, int32_t max_active, gls_server **out);
+int gls_server_submit(gls_server *s, int32_t nPIs, int32_t nPOs, int32_t nAnds, const uint32_t *fanin0,
+                      const uint32_t *fanin1, const uint32_t *pos, int32_t lit_convention, const char *script,
+                      int64_t *job);
+/* Blocks until the job is done (timeout_ms < 0: no limit; GLS_ERR_TIMEOUT
+ * when it runs out).  info may be NULL. */
+int gls_server_wait(gls_server *s, int64_t job, double timeout_ms, gls_job_info *info);
+/* The result of a done job: fanin arrays of info.nPIs + 1 + info.nAnds
+ * entries, pos of info.nPOs. */
+int gls_server_result(gls_server *s, int64_t job, uint32_t *fanin0, uint32_t *fanin1, uint32_t *pos,
+                      int32_t lit_convention);
+int gls_server_release(gls_server *s, int64_t job);
+const char *gls_server_last_error(gls_server *s);
+/* Finishes every submitted job, then stops the worker. */
+void gls_server_destroy(gls_server *s);
 
 /* ---- stage-level entry points (parity tests and tools) ----------------- */
 /* Level-major renumbering of a strashed AIG as done before rw

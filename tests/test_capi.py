@@ -46,6 +46,16 @@ def test_batch_needs_a_device(host_ctx):
         host_ctx.batch_strash([a])
 
 
+def test_server_needs_a_device(host_ctx):
+    try:
+        culs_amd.Context(0).close()
+        pytest.skip("a GPU is present")
+    except GlsError:
+        pass
+    with pytest.raises(GlsError):
+        culs_amd.Server(0)
+
+
 def test_aiger_round_trip(host_ctx, tmp_path):
     a = host_ctx.gen_multiplier(6, 0.1, 3)
     p = str(tmp_path / "m.aig")

@@ -92,3 +92,18 @@ def test_batch_rejects_repeated_member(gpu_ctx):
     with pytest.raises(culs_amd.GlsError):
         gpu_ctx.batch_rewrite([a, a])
     assert gpu_ctx.batch_rewrite([]) == []
+
+
+def test_batch_balance_matches_alone_and_oracle(gpu_ctx):
+    log("batch balance")
+    aigs = mixed(gpu_ctx, 17, 4)
+    raw = [a.arrays() for a in aigs]
+    gpu_ctx.batch_balance(aigs, update_level=False)
+    for g, a in zip(raw, aigs):
+        alone = gpu_ctx.from_arrays(*g).balance(update_level=False)
+        assert same(a.arrays(), alone.arrays())
+        assert same(a.arrays(), O.balance(*g))
+        assert a.stats().levels == -1
+    gpu_ctx.batch_strash(aigs)
+    for g, a in zip(raw, aigs):
+        assert same(a.arrays(), O.strash(*O.balance(*g))[0])
