@@ -129,7 +129,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn = nullptr,
                 uint32_t **tagOut = nullptr);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
-                       uint32_t *d_cuts, int32_t *d_nref, int mode);
+                       uint32_t *d_cuts, int32_t *d_nref, int mode, const uint8_t *sigBits = nullptr);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,
                  int useZeros, int32_t *d_best);
 void dev_graph_free(gls_ctx *ctx, DevGraph &g);

@@ -63,6 +63,17 @@ struct DCut {
     int32_t nLeaves, used;
 };
 
+// Leaf signature of the reference (rewrite.cu:68, 262: bit leaf & 31).  In a
+// batched pass sigBits holds each node's bit as it would be in its own AIG
+// (its id there & 31), so that the prefilter of rewrite.cu:283 sees the same
+// signatures as when that AIG is rewritten alone.
+__device__ __forceinline__ uint32_t cut_sign_of(const DCut &c, const uint8_t *__restrict__ sigBits) {
+    uint32_t s = 0;
+    for (int i = 0; i < 4; i++)
+        if (i < c.nLeaves) s |= 1u << (sigBits ? sigBits[c.leaves[i]] : (c.leaves[i] & 31));
+    return s;
+}
+
 __device__ __forceinline__ uint32_t cut_sign(const DCut &c) {
     uint32_t s = 0;
     for (int i = 0; i < 4; i++)
@@ -322,7 +333,8 @@ enum : int { REC_PLAIN = 0, REC_TAGGED = 1 };
 template <int MODE>
 __device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
                                          const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
-                                         uint32_t *cuts, unsigned long long *tcuts, uint32_t *err) {
+                                         uint32_t *cuts, unsigned long long *tcuts, uint32_t *err,
+                                         const uint8_t *__restrict__ sigBits) {
     const uint32_t l0 = f0[idx], l1 = f1[idx];
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
     DCut a, b;
@@ -350,7 +362,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     m.leaves[0] = m.leaves[1] = m.leaves[2] = m.leaves[3] = 0;
     m.nLeaves = 0; m.tt = 0; m.value = 0; m.used = 0;
     mm = m;
-    if (a.used && b.used && __popc(cut_sign(a) | cut_sign(b)) <= 4) {
+    if (a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4) {
         int ok = a.nLeaves >= b.nLeaves ? merge_ordered(a, b, m) : merge_ordered(b, a, m);
         if (!ok) {
             status = C_FAIL;
@@ -437,10 +449,11 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
 __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_t count,
                                                            const uint32_t *__restrict__ f0,
                                                            const uint32_t *__restrict__ f1,
-                                                           const int32_t *__restrict__ nRef, uint32_t *cuts) {
+                                                           const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                           const uint8_t *__restrict__ sigBits) {
     const int k = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     if (k >= count) return;    // uniform per wavefront
-    cut_node<REC_PLAIN>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
+    cut_node<REC_PLAIN>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
 }
 
 // All levels in one cooperative launch: the wavefronts of the grid stride over
@@ -448,13 +461,15 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_
 __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t l1, const int32_t *__restrict__ starts,
                                                             const uint32_t *__restrict__ f0,
                                                             const uint32_t *__restrict__ f1,
-                                                            const int32_t *__restrict__ nRef, uint32_t *cuts) {
+                                                            const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                            const uint8_t *__restrict__ sigBits) {
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int nw = gridDim.x * (CUTW_BLOCK / 64);
     const int gw = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     for (int l = l0; l < l1; l++) {
         const int s = starts[l], e = starts[l + 1];
-        for (int v = s + gw; v < e; v += nw) cut_node<REC_PLAIN>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr);
+        for (int v = s + gw; v < e; v += nw)
+            cut_node<REC_PLAIN>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
         grid.sync();
     }
 }
@@ -470,7 +485,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_dataflow(int32_t first, int
                                                               const uint32_t *__restrict__ f1,
                                                               const int32_t *__restrict__ nRef,
                                                               unsigned long long *tcuts, uint32_t *ticket,
-                                                              uint32_t *err) {
+                                                              uint32_t *err, const uint8_t *__restrict__ sigBits) {
     const int lane = threadIdx.x & 63;
     for (;;) {
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -479,7 +494,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_dataflow(int32_t first, int
         t = (uint32_t)__shfl((int)t, 0);
         const int v = first + (int)t;
         if (v >= last) break;
-        if (!cut_node<REC_TAGGED>(v, lane, f0, f1, nRef, nullptr, tcuts, err)) break;
+        if (!cut_node<REC_TAGGED>(v, lane, f0, f1, nRef, nullptr, tcuts, err, sigBits)) break;
     }
 }
 
@@ -1031,6 +1046,62 @@ struct Timer {
 
 } // namespace
 
+// Signature bits of a batched pass: every node's id in its own AIG, & 31.
+// Nodes 1..n-1 sorted stably by tag keep their id order inside each tag; in
+// the reordered numbering an AIG's PIs come first, then its ANDs level-major
+// (ties by id), exactly as its own reordered numbering, so the rank inside the
+// tag group + 1 is that id.
+__global__ void k_sig_keys(int32_t n, const uint32_t *__restrict__ tag, uint32_t *keys, uint32_t *vals,
+                           uint32_t *hist) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t u = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += stride) {
+        keys[u - 1] = tag[u];
+        vals[u - 1] = (uint32_t)u;
+        atomicAdd(&hist[tag[u]], 1u);
+    }
+}
+
+__global__ void k_sig_bits(int32_t m, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ svals,
+                           const uint32_t *__restrict__ start, uint8_t *sigBits) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < m; q += stride)
+        sigBits[svals[q]] = (uint8_t)((1u + (uint32_t)(q - start[skeys[q]])) & 31u);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sigBits[0] = 0;
+}
+
+static int sig_bits_for_tags(gls_ctx *ctx, int32_t n, const uint32_t *tag, uint8_t *sigBits) {
+    hipStream_t st = ctx->stream;
+    const int32_t m = n - 1;
+    if (m <= 0) return GLS_OK;
+    Tmp t{ctx};
+    uint32_t *mx = t.get<uint32_t>(1);
+    size_t tb = 0;
+    GLS_HIP(ctx, rocprim::reduce(nullptr, tb, tag, mx, 0u, (size_t)n, rocprim::maximum<uint32_t>(), st));
+    void *tmp = t.get<uint8_t>(tb);
+    if (!mx || !tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "signature bits: allocation");
+    GLS_HIP(ctx, rocprim::reduce(tmp, tb, tag, mx, 0u, (size_t)n, rocprim::maximum<uint32_t>(), st));
+    uint32_t maxTag = 0;
+    GLS_TRY(stream_copy(ctx, &maxTag, mx, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t *keys = t.get<uint32_t>(m), *vals = t.get<uint32_t>(m), *skeys = t.get<uint32_t>(m);
+    uint32_t *svals = t.get<uint32_t>(m), *hist = t.get<uint32_t>(maxTag + 2), *start = t.get<uint32_t>(maxTag + 2);
+    if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "signature bits: allocation");
+    GLS_HIP(ctx, hipMemsetAsync(hist, 0, sizeof(uint32_t) * (maxTag + 2), st));
+    k_sig_keys<<<grid_for(m), kBlock, 0, st>>>(n, tag, keys, vals, hist);
+    size_t tb2 = 0;
+    const uint32_t eb = bits_for(maxTag + 1);
+    GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb2, keys, skeys, vals, svals, (size_t)m, 0u, eb, st));
+    size_t tb3 = 0;
+    GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb3, hist, start, 0u, (size_t)maxTag + 1, rocprim::plus<uint32_t>(), st));
+    void *tmp2 = t.get<uint8_t>(std::max(tb2, tb3));
+    if (!tmp2) return gls_fail(ctx, GLS_ERR_NOMEM, "signature bits: sort storage");
+    GLS_HIP(ctx, rocprim::radix_sort_pairs(tmp2, tb2, keys, skeys, vals, svals, (size_t)m, 0u, eb, st));
+    GLS_HIP(ctx, rocprim::exclusive_scan(tmp2, tb3, hist, start, 0u, (size_t)maxTag + 1, rocprim::plus<uint32_t>(), st));
+    k_sig_bits<<<grid_for(m), kBlock, 0, st>>>(m, skeys, svals, start, sigBits);
+    GLS_HIP(ctx, hipGetLastError());
+    GLS_HIP(ctx, hipStreamSynchronize(st));
+    return GLS_OK;
+}
+
 int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart,
                 const uint32_t *tagIn, uint32_t **tagOut) {
     const int32_t n = in.nObjs, m = n - 1;
@@ -1100,7 +1171,7 @@ static const bool kSampleLaunches = [] {
 }();
 
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
-                       int32_t *d_nref, int mode) {
+                       int32_t *d_nref, int mode, const uint8_t *sigBits) {
     hipStream_t st = ctx->stream;
     const int32_t n = g.nObjs;
     ctx->ksamp_n = 0;
@@ -1127,7 +1198,8 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         GLS_HIP(ctx, hipMemsetAsync(ctr, 0, sizeof(uint32_t) * 4, st));
         if (g.nPIs > 0)
             k_cuts_pi_tagged<<<blocks((int64_t)g.nPIs * NCUT, kBlock), kBlock, 0, st>>>(g.nPIs, d_nref, tcuts);
-        k_cuts_dataflow<<<G, CUTW_BLOCK, 0, st>>>(g.nPIs + 1, n, g.f0, g.f1, d_nref, tcuts, &ctr[0], &ctr[1]);
+        k_cuts_dataflow<<<G, CUTW_BLOCK, 0, st>>>(g.nPIs + 1, n, g.f0, g.f1, d_nref, tcuts, &ctr[0], &ctr[1],
+                                                   sigBits);
         k_untag_cuts<<<grid_for((int64_t)nslots), kBlock, 0, st>>>((int64_t)nslots, tcuts, d_cuts);
         GLS_HIP(ctx, hipGetLastError());
         uint32_t hctr[2] = {0, 0};
@@ -1162,7 +1234,8 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         const uint32_t *a0 = g.f0, *a1 = g.f1;
         const int32_t *ar = d_nref;
         uint32_t *ac = d_cuts;
-        void *args[] = {&l0, &l1, &d_starts, &a0, &a1, &ar, &ac};
+        const uint8_t *sb = sigBits;
+        void *args[] = {&l0, &l1, &d_starts, &a0, &a1, &ar, &ac, &sb};
         hipError_t e = hipLaunchCooperativeKernel((const void *)k_cuts_levels, dim3(G), dim3(CUTW_BLOCK), args, 0, st);
         // the pool block is reused only by later work on the same stream
         pool_put(ctx, d_starts);
@@ -1183,11 +1256,11 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                 }
                 hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTW_BLOCK), 0, st, ctx->kev[k], ctx->kev[k + 1], 0,
                                       s, e - s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
-                                      (const int32_t *)d_nref, d_cuts);
+                                      (const int32_t *)d_nref, d_cuts, sigBits);
                 ctx->ksamp_units.push_back(e - s);
                 ctx->ksamp_n++;
             } else {
-                k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts);
+                k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
         }
     }
@@ -1265,7 +1338,14 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         unsigned long long *d_gain = t.get<unsigned long long>(1);
         if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n); }
         GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 16, st));
-        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS)));
+        uint8_t *sigBits = nullptr;
+        if (tagged) {
+            sigBits = t.get<uint8_t>(n);
+            if (!sigBits) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: signature bits"); }
+            GLS_TRY(sig_bits_for_tags(ctx, n, tagR, sigBits));
+        }
+        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS),
+                                   sigBits));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};
         GLS_TRY(build_table(ctx, t, g, gv, d_err));

@@ -644,9 +644,15 @@ int orc_reorder(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPO
 /* ------------------------------------------------------------------ */
 typedef struct { uint32_t leaves[4]; uint32_t tt, value; int nLeaves, used; } Cut;
 
+/* Signature bit of each node (rewrite.cu:68, 262: leaf & 31).  A batched pass
+ * over a union of AIGs uses every node's bit in its own AIG instead; tests set
+ * that table here (indexed by the ids of the reordered graph). */
+static const uint8_t *g_sig_bits = NULL;
+void orc_set_sig_bits(const uint8_t *bits) { g_sig_bits = bits; }
+
 static uint32_t cut_sign(const Cut *c) {
     uint32_t s = 0;
-    for (int i = 0; i < c->nLeaves; i++) s |= 1u << (c->leaves[i] & 31);
+    for (int i = 0; i < c->nLeaves; i++) s |= 1u << (g_sig_bits ? g_sig_bits[c->leaves[i]] : (c->leaves[i] & 31));
     return s;
 }
 

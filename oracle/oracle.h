@@ -60,6 +60,11 @@ int orc_rw(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs
            const uint32_t *pos, int useZeros, int cap, uint32_t *of0, uint32_t *of1,
            uint32_t *opos, int64_t *stats);
 
+/* Signature bits of the cut prefilter, indexed by node id of the reordered
+ * graph (NULL: id & 31, the reference's rule).  Models a batched pass, where
+ * every node keeps the bit of its id in its own AIG. */
+void orc_set_sig_bits(const uint8_t *bits);
+
 /* The graph orc_rw hands to its final strash: the reordered input with the
  * selected roots substituted and the new nodes appended (ids n..N-1, fanins
  * possibly above their own id's level order but acyclic).  Returns N. */
