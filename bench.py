@@ -332,16 +332,24 @@ def bench_serve(a):
     served_span = time.perf_counter() - start
     lat_s = [(i["done_ms"] - i["submit_ms"]) for i in infos]
     served_out = [srv.result(j) for j in ids]
-    srv.close()
     log("served: makespan %.2fs p50 %.1f ms p99 %.1f ms" % (served_span, pct(lat_s, 50), pct(lat_s, 99)))
+    # capacity: every job submitted at once
+    start = time.perf_counter()
+    ids2 = [srv.submit(g, script) for g in jobs]
+    for j in ids2:
+        srv.wait(j)
+    sat_span = time.perf_counter() - start
+    srv.close()
+    log("served, all at once: %.2fs" % sat_span)
 
     # plain loop with a host round trip per command
-    lat_l, loop_out = [], []
+    lat_l, loop_out, busy = [], [], 0.0
     start = time.perf_counter()
     for k, g in enumerate(jobs):
         d = arrivals[k] - (time.perf_counter() - start)
         if d > 0:
             time.sleep(d)
+        t_job = time.perf_counter()
         cur = g
         for cmd in ("b", "rw", "rwz", "b"):
             x = ctx.from_arrays(*cur)
@@ -352,6 +360,7 @@ def bench_serve(a):
             cur = x.arrays()
             x.free()
         loop_out.append(cur)
+        busy += time.perf_counter() - t_job
         lat_l.append((time.perf_counter() - start - arrivals[k]) * 1e3)
     loop_span = time.perf_counter() - start
     log("loop: makespan %.2fs p50 %.1f ms p99 %.1f ms" % (loop_span, pct(lat_l, 50), pct(lat_l, 99)))
@@ -373,6 +382,9 @@ def bench_serve(a):
                  "makespan_s": round(loop_span, 3), "jobs_per_s": round(len(jobs) / loop_span, 3),
                  "value": round(n_in / loop_span / 1e6, 4)},
         "throughput_vs_loop": round(loop_span / served_span, 3),
+        "capacity": {"served_all_at_once_s": round(sat_span, 3), "loop_busy_s": round(busy, 3),
+                     "served_jobs_per_s": round(len(jobs) / sat_span, 3),
+                     "loop_jobs_per_s": round(len(jobs) / busy, 3), "ratio": round(busy / sat_span, 3)},
         "identical_to_loop": bool(identical),
     }
     print(json.dumps(res), flush=True)

@@ -53,6 +53,7 @@ constexpr int COVER_THREADS = 16384;  // threads of a cover-discovery launch (ea
 // the size limit is checked before an expansion, so a cover can end up to a
 // few inputs above BAL_MAX_COVER: room for that
 constexpr int COVER_SCRATCH = BAL_MAX_COVER + 64;
+constexpr int COVER_LEVELS_PER_CHECK = 32;  // cover-discovery levels launched per host check
 constexpr uint32_t UNSET = 0xFFFFFFFFu;
 constexpr unsigned long long EMPTY64 = ~0ull;
 
@@ -208,8 +209,14 @@ __device__ uint32_t cover_dfs(const CoverArgs &a, uint32_t v, uint32_t *in) {
 
 // one level of the root sweep: record the level, compute covers seen for the
 // first time, and put their AND inputs on the next frontier (each once)
-__global__ void k_cover_level(CoverArgs a, const uint32_t *__restrict__ front, uint32_t frontLen, uint32_t level,
-                              uint32_t *next, uint32_t *nextLen) {
+// flen[level] = size of the level's frontier, kept on the device so that
+// several levels are launched per host check; level L's frontier is in
+// fronts[L & 1].
+__global__ void k_cover_level(CoverArgs a, uint32_t *front0, uint32_t *front1, uint32_t *flen, uint32_t level) {
+    const uint32_t frontLen = flen[level];
+    const uint32_t *front = (level & 1u) ? front1 : front0;
+    uint32_t *next = (level & 1u) ? front0 : front1;
+    uint32_t *nextLen = &flen[level + 1];
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     uint32_t *in = a.scratch + (size_t)tid * COVER_SCRATCH;
     for (int64_t k = tid; k < frontLen; k += stride) {
@@ -571,7 +578,8 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     uint32_t *nFo = t.get<uint32_t>(n), *coverOff = t.get<uint32_t>(n), *coverLen = t.get<uint32_t>(n);
     uint32_t *lastLevel = t.get<uint32_t>(n), *stamp = t.get<uint32_t>(n);
     uint32_t *frontA = t.get<uint32_t>(n), *frontB = t.get<uint32_t>(n);
-    uint32_t *ctr = t.get<uint32_t>(8);   // [0] front len, [1] pool next, [2] err, [4..7] reconstruction counter
+    uint32_t *ctr = t.get<uint32_t>(8);   // [1] pool next, [2] err, [4..7] reconstruction counter
+    uint32_t *flen = t.get<uint32_t>((size_t)n + 2 + COVER_LEVELS_PER_CHECK);   // frontier size per level
     const uint64_t poolCap = 2ull * (uint64_t)n + 64;   // cover inputs are distinct fanin edges
     uint32_t *pool = t.get<uint32_t>(poolCap);
     uint32_t *scratch = t.get<uint32_t>((size_t)COVER_THREADS * COVER_SCRATCH);
@@ -581,21 +589,26 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     GLS_HIP(ctx, hipMemsetAsync(lastLevel, 0xFF, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(stamp, 0, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(ctr, 0, sizeof(uint32_t) * 8, st));
+    GLS_HIP(ctx, hipMemsetAsync(flen, 0, sizeof(uint32_t) * ((size_t)n + 2 + COVER_LEVELS_PER_CHECK), st));
     k_fanouts<<<nblocks(n), kB, 0, st>>>(n, nPIs, in.f0, in.f1, nPOs, in.pos, nFo);
-    if (nPOs > 0) k_po_frontier<<<nblocks(nPOs), kB, 0, st>>>(nPOs, nPIs, in.pos, stamp, frontA, &ctr[0]);
+    if (nPOs > 0) k_po_frontier<<<nblocks(nPOs), kB, 0, st>>>(nPOs, nPIs, in.pos, stamp, frontA, &flen[0]);
     GLS_HIP(ctx, hipGetLastError());
     CoverArgs ca{in.f0, in.f1, nFo, nPIs, coverOff, coverLen, pool, &ctr[1], poolCap, scratch, lastLevel, stamp,
                  &ctr[2]};
-    uint32_t frontLen = 0, levels = 0;
-    GLS_TRY(stream_copy(ctx, &frontLen, &ctr[0], sizeof(uint32_t), hipMemcpyDeviceToHost));
-    while (frontLen > 0) {
-        GLS_HIP(ctx, hipMemsetAsync(&ctr[0], 0, sizeof(uint32_t), st));
-        const int g = (int)std::min<int64_t>((frontLen + kB - 1) / kB, COVER_THREADS / kB);
-        k_cover_level<<<g, kB, 0, st>>>(ca, frontA, frontLen, levels, frontB, &ctr[0]);
+    // levels are launched COVER_LEVELS_PER_CHECK at a time; a launch past the
+    // last level finds an empty frontier and does nothing
+    uint32_t levels = 0;
+    for (uint32_t base = 0;; base += COVER_LEVELS_PER_CHECK) {
+        if ((uint64_t)base > (uint64_t)n) return gls_fail(ctx, GLS_ERR_INTERNAL, "balance: cover levels do not end");
+        for (uint32_t l = base; l < base + COVER_LEVELS_PER_CHECK; l++)
+            k_cover_level<<<COVER_THREADS / kB, kB, 0, st>>>(ca, frontA, frontB, flen, l);
         GLS_HIP(ctx, hipGetLastError());
-        std::swap(frontA, frontB);
-        levels++;
-        GLS_TRY(stream_copy(ctx, &frontLen, &ctr[0], sizeof(uint32_t), hipMemcpyDeviceToHost));
+        uint32_t h[COVER_LEVELS_PER_CHECK + 1];
+        GLS_TRY(stream_copy(ctx, h, &flen[base], sizeof(h), hipMemcpyDeviceToHost));
+        int first0 = -1;
+        for (int i = 0; i <= COVER_LEVELS_PER_CHECK; i++)
+            if (h[i] == 0) { first0 = i; break; }
+        if (first0 >= 0) { levels = base + (uint32_t)first0; break; }
     }
     uint32_t herr = 0, poolUsed = 0;
     GLS_TRY(stream_copy(ctx, &herr, &ctr[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -651,14 +664,15 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
         }
     }
     const uint32_t batchLimit = tagged ? 0xFFFFFFFFu : (uint32_t)BAL_BATCH;
-    // maximum cover length
+    // cover lengths: the workspace width, and the steps a batch needs (a step
+    // takes two inputs of every unfinished cover and gives back at most one)
     uint32_t maxLen = 1;
-    {
-        std::vector<uint32_t> hl(n);
-        GLS_TRY(stream_copy(ctx, hl.data(), coverLen, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        for (int32_t v = 0; v < n; v++)
-            if (hl[v] != UNSET && hl[v] > maxLen) maxLen = hl[v];
-    }
+    std::vector<uint32_t> hl(n), hroots(nRoots);
+    GLS_TRY(stream_copy(ctx, hl.data(), coverLen, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    for (int32_t v = 0; v < n; v++)
+        if (hl[v] != UNSET && hl[v] > maxLen) maxLen = hl[v];
+    if (nRoots > 0)
+        GLS_TRY(stream_copy(ctx, hroots.data(), rootsSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
 
     // ---- phase 2: reconstruction
     uint64_t cap = 1024;
@@ -709,12 +723,11 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
         // one wide level, in batches, one kernel per phase
         for (uint32_t b0 = starts[l]; b0 < starts[l + 1]; b0 += std::min<uint32_t>(starts[l + 1] - b0, batchLimit)) {
             const uint32_t nb = std::min<uint32_t>(starts[l + 1] - b0, batchLimit);
+            uint32_t steps = 0;
+            for (uint32_t i = b0; i < b0 + nb; i++) steps = std::max(steps, hl[hroots[i]] > 1 ? hl[hroots[i]] - 1 : 0u);
             k_rec_gather<<<nblocks(nb), kB, 0, st>>>(ra, b0, nb);
-            for (;;) {
+            for (uint32_t stp = 0; stp < steps; stp++) {
                 k_rec_prepare<<<nblocks(nb), kB, 0, st>>>(ra, nb);
-                uint32_t any = 0;
-                GLS_TRY(stream_copy(ctx, &any, &ctr[6], sizeof(uint32_t), hipMemcpyDeviceToHost));
-                if (!any) break;
                 k_rec_insert<<<nblocks(nb), kB, 0, st>>>(ra, nb);
                 k_rec_retrieve<<<nblocks(nb), kB, 0, st>>>(ra, nb);
                 k_rec_add_back<<<nblocks(nb), kB, 0, st>>>(ra, b0, nb);
