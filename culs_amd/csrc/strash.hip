@@ -132,10 +132,10 @@ __device__ __forceinline__ uint32_t map_fanin(const WaveArgs &a, uint32_t lit, u
 // minimum, so the representative of duplicates is the smallest (wave, id).
 // Returns true for a non-trivial node (slot and level filled, rl not yet).
 template <int SCOPE>
-__device__ __forceinline__ bool hash_node(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t &slotOut,
-                                          uint32_t &levOut) {
+__device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t raw0,
+                                               uint32_t raw1, uint32_t &slotOut, uint32_t &levOut) {
     uint32_t la, lb;
-    uint32_t l0 = map_fanin<SCOPE>(a, a.f0[v], la), l1 = map_fanin<SCOPE>(a, a.f1[v], lb);
+    uint32_t l0 = map_fanin<SCOPE>(a, raw0, la), l1 = map_fanin<SCOPE>(a, raw1, lb);
     if (l0 > l1) { uint32_t t = l0; l0 = l1; l1 = t; t = la; la = lb; lb = t; }
     a.wave[v] = k;
     uint32_t r = INVALID32, rlev = 0;
@@ -168,11 +168,16 @@ __device__ __forceinline__ bool hash_node(const WaveArgs &a, uint32_t k, uint32_
     return true;
 }
 
+template <int SCOPE>
+__device__ __forceinline__ bool hash_node(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t &slotOut,
+                                          uint32_t &levOut) {
+    return hash_node_lits<SCOPE>(a, k, v, a.f0[v], a.f1[v], slotOut, levOut);
+}
+
 // Releases the fanouts of v whose last pending fanin is v.  Up to four
 // decrements are in flight before any result is inspected.
 template <int SCOPE, class Emit>
-__device__ __forceinline__ void release(const WaveArgs &a, uint32_t v, Emit emit) {
-    const uint32_t e0 = a.adjOff[v], ee = a.adjOff[v + 1];
+__device__ __forceinline__ void release_range(const WaveArgs &a, uint32_t e0, uint32_t ee, Emit emit) {
     for (uint32_t e = e0; e < ee; e += 4) {
         uint32_t w[4], r[4];
 #pragma unroll
@@ -184,6 +189,11 @@ __device__ __forceinline__ void release(const WaveArgs &a, uint32_t v, Emit emit
         for (int j = 0; j < 4; j++)
             if (e + j < ee && r[j] == 1u) emit(w[j]);
     }
+}
+
+template <int SCOPE, class Emit>
+__device__ __forceinline__ void release(const WaveArgs &a, uint32_t v, Emit emit) {
+    release_range<SCOPE>(a, a.adjOff[v], a.adjOff[v + 1], emit);
 }
 
 template <int SCOPE>
@@ -218,13 +228,19 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
 // workgroup barriers instead of kernel boundaries, the frontier lives in LDS,
 // atomics are workgroup-scoped (performed in the local L2), and each wave's
 // representatives are resolved right after its inserts, so the next wave maps
-// a fanin with a single load.  Runs while the frontier is non-empty and at
-// most `thresh` (<= FRONT_MAX) wide; *kOut = next wave to run.
+// a fanin with a single load.  A node entering the next frontier has its fanin
+// literals and fanout range fetched into LDS at once, by the thread that
+// released it: the next wave then starts with the representative reads and the
+// fanout-counter atomics instead of first fetching them.  Runs while the
+// frontier is non-empty and at most `thresh` (<= FRONT_MAX) wide; *kOut = next
+// wave to run.
 constexpr int WG1 = 1024;
-constexpr int FRONT_MAX = 4096;
+constexpr int FRONT_MAX = 2048;
 constexpr int WG1_ITEMS = FRONT_MAX / WG1;
 __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh, WaveArgs a, uint32_t *kOut) {
     __shared__ uint32_t s_front[2][FRONT_MAX];
+    __shared__ uint32_t s_f0[2][FRONT_MAX], s_f1[2][FRONT_MAX];      // fanin literals
+    __shared__ uint32_t s_e0[2][FRONT_MAX], s_e1[2][FRONT_MAX];      // fanout range in adj
     __shared__ uint32_t s_next;
     const uint32_t tid = threadIdx.x;
     uint32_t k = k0;
@@ -233,7 +249,14 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
     // the previous wave may come from the grid-wide kernel, unresolved
     for (uint32_t i = tid; i < pn; i += WG1) resolve<__HIP_MEMORY_SCOPE_WORKGROUP>(a, a.order[ps + i]);
     if (cn <= thresh)
-        for (uint32_t i = tid; i < cn; i += WG1) s_front[0][i] = a.order[cs + i];
+        for (uint32_t i = tid; i < cn; i += WG1) {
+            const uint32_t v = a.order[cs + i];
+            s_front[0][i] = v;
+            s_f0[0][i] = a.f0[v];
+            s_f1[0][i] = a.f1[v];
+            s_e0[0][i] = a.adjOff[v];
+            s_e1[0][i] = a.adjOff[v + 1];
+        }
     __syncthreads();
     int buf = 0;
     uint32_t last = 0;
@@ -253,18 +276,25 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             if (i < cn) {
                 const uint32_t v = s_front[buf][i];
                 myV[j] = v;
-                myNT[j] = hash_node<__HIP_MEMORY_SCOPE_WORKGROUP>(a, k, v, mySlot[j], myLev[j]);
+                myNT[j] = hash_node_lits<__HIP_MEMORY_SCOPE_WORKGROUP>(a, k, v, s_f0[buf][i], s_f1[buf][i],
+                                                                         mySlot[j], myLev[j]);
             }
         }
 #pragma unroll
         for (int j = 0; j < 2 * WG1_ITEMS; j++) {
             const uint32_t i = tid + (uint32_t)j * WG1;
             if (i >= cn && i < 2 * cn) {
-                const uint32_t v = s_front[buf][i - cn];
-                release<__HIP_MEMORY_SCOPE_WORKGROUP>(a, v, [&](uint32_t w) {
+                const uint32_t q = i - cn;
+                release_range<__HIP_MEMORY_SCOPE_WORKGROUP>(a, s_e0[buf][q], s_e1[buf][q], [&](uint32_t w) {
                     const uint32_t p = atomicAdd(&s_next, 1u);
-                    if (p < FRONT_MAX) s_front[buf ^ 1][p] = w;
                     a.order[nb + p] = w;
+                    if (p < FRONT_MAX) {
+                        s_front[buf ^ 1][p] = w;
+                        s_f0[buf ^ 1][p] = a.f0[w];
+                        s_f1[buf ^ 1][p] = a.f1[w];
+                        s_e0[buf ^ 1][p] = a.adjOff[w];
+                        s_e1[buf ^ 1][p] = a.adjOff[w + 1];
+                    }
                 });
             }
         }
