@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
                     help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
+    ap.add_argument("--stage", choices=["rw", "cuts"], default="rw",
+                    help="cuts: time only reorder + cut enumeration (gls_enumerate_cuts), for counter runs")
     ap.add_argument("--workload", choices=["rw10m", "batch", "serve"], default="rw10m",
                     help="rw10m: the headline line; batch: 4096 mixed AIGs packed into one arena vs one at a "
                          "time; serve: a Poisson stream of `b; rw; rwz; b` jobs, served vs a per-command loop")
@@ -106,8 +108,19 @@ def main():
 
     def step():
         c = src.copy()
+        if a.stage == "cuts":
+            c.reorder()
+            c.enumerate_cuts()
+            return c
         c.rewrite(cut_mode=a.cut_mode)
         return c
+
+    if a.stage == "cuts":
+        for i in range(a.steps):
+            t = time.perf_counter()
+            step().free()
+            log("cuts step %d: %.1f ms" % (i, (time.perf_counter() - t) * 1e3))
+        return
 
     for _ in range(a.warmup):
         step().free()
