@@ -90,6 +90,36 @@ __global__ void k_kahn_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict_
     }
 }
 
+// Pre-inserts the key of every live non-trivial AND as read from its own
+// fanin literals, leaving the value empty; slotOf[v] = its slot (NO_SLOT for a
+// trivial key).  A node whose fanins turn out to be their own representatives
+// keeps this key, so in the sweep it only lowers the slot's value and skips
+// the probe.
+constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
+__global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                            const uint8_t *__restrict__ dead, unsigned long long *ht, uint64_t mask, uint32_t *slotOf,
+                            uint32_t *err) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (dead[v]) continue;
+        uint32_t l0 = f0[v], l1 = f1[v];
+        if (l0 > l1) { const uint32_t t = l0; l0 = l1; l1 = t; }
+        if (l0 == l1 || l0 == (l1 ^ 1u) || l0 < 2u) { slotOf[v] = NO_SLOT; continue; }
+        const uint64_t key = and_key(l0, l1);
+        uint64_t s = mix64(key) & mask;
+        uint32_t slot = NO_SLOT;
+        for (uint64_t probe = 0; probe <= mask; ++probe) {
+            unsigned long long prev = KEY_EMPTY;
+            __hip_atomic_compare_exchange_strong(&ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
+            s = (s + 1) & mask;
+        }
+        if (slot == NO_SLOT) atomicOr(err, 1u);
+        slotOf[v] = slot;
+    }
+}
+
 struct WaveArgs {
     const uint32_t *f0, *f1;
     int32_t nPIs;
@@ -149,6 +179,16 @@ __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, ui
     }
     levOut = 1u + (la > lb ? la : lb);
     const uint64_t key = and_key(l0, l1), val = (static_cast<uint64_t>(k) << 32) | v;
+    {
+        // fanins that are their own representatives: the pre-inserted key holds
+        const uint32_t r0 = raw0 < raw1 ? raw0 : raw1, r1 = raw0 < raw1 ? raw1 : raw0;
+        const uint32_t pre = a.slotOf[v];
+        if (l0 == r0 && l1 == r1 && pre != NO_SLOT) {
+            __hip_atomic_fetch_min(&a.ht[2 * (size_t)pre + 1], (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
+            slotOut = pre;
+            return true;
+        }
+    }
     uint64_t s = mix64(key) & a.mask;
     int64_t slot = -1;
     for (uint64_t probe = 0; probe <= a.mask; ++probe) {
@@ -453,8 +493,9 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     uint32_t *slotOf = t.get<uint32_t>(n), *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
     uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
     SweepCtl *ctl = t.get<SweepCtl>(1);
+    // room for every live node's own key plus a changed key per node
     uint64_t cap = 1024;
-    while (cap < 2ull * (uint64_t)n) cap <<= 1;
+    while (cap < 4ull * (uint64_t)n) cap <<= 1;
     unsigned long long *ht = t.get<unsigned long long>(2 * cap);
     if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
         !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !slotOf ||
@@ -498,6 +539,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
     k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj);
     k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
+    k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotOf, &d_misc[1]);
     GLS_HIP(ctx, hipGetLastError());
 
     // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
