@@ -446,25 +446,11 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     return true;
 }
 
-// One launch per level.  The grid has one workgroup more than the level
-// needs; its first wavefront only reads the next level's fanin literals (one
-// load per 128-byte line, into a sink word), alongside the node work, so that
-// the next launch finds them in the device-wide last-level cache.
 __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_level(int32_t start, int32_t count,
                                                            const uint32_t *__restrict__ f0,
                                                            const uint32_t *__restrict__ f1,
                                                            const int32_t *__restrict__ nRef, uint32_t *cuts,
-                                                           const uint8_t *__restrict__ sigBits, int32_t nextCount,
-                                                           uint32_t *sink) {
-    if (blockIdx.x == gridDim.x - 1) {
-        if (threadIdx.x < 64 && nextCount > 0) {
-            uint32_t acc = 0;
-            const int32_t next = start + count;
-            for (int32_t i = (int32_t)threadIdx.x * 32; i < nextCount; i += 64 * 32) acc ^= f0[next + i] ^ f1[next + i];
-            sink[threadIdx.x] = acc;
-        }
-        return;
-    }
+                                                           const uint8_t *__restrict__ sigBits) {
     const int k = blockIdx.x * (CUTW_BLOCK / 64) + (threadIdx.x >> 6);
     if (k >= count) return;    // uniform per wavefront
     cut_node<REC_PLAIN>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
@@ -1255,13 +1241,10 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         pool_put(ctx, d_starts);
         if (e != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cooperative cut launch: %s", hipGetErrorString(e));
     } else {
-        uint32_t *sink = pool_get<uint32_t>(ctx, 64);
-        if (!sink) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: sink");
         for (int l = 1; l < L; l++) {
             int32_t s = levelStart[l], e = levelStart[l + 1];
             if (e <= s) continue;
-            const int32_t nextCount = l + 1 < L ? levelStart[l + 2] - e : 0;
-            const dim3 grid(blocks(e - s, CUTW_BLOCK / 64) + 1);   // + the prefetch workgroup
+            const dim3 grid(blocks(e - s, CUTW_BLOCK / 64));
             if (kSampleLaunches && l % KSAMPLE_EVERY == 0) {
                 // this launch carries events on its own dispatch: their elapsed
                 // time is the kernel's execution time, as a profiler reports it
@@ -1273,15 +1256,13 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                 }
                 hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTW_BLOCK), 0, st, ctx->kev[k], ctx->kev[k + 1], 0,
                                       s, e - s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
-                                      (const int32_t *)d_nref, d_cuts, sigBits, nextCount, sink);
+                                      (const int32_t *)d_nref, d_cuts, sigBits);
                 ctx->ksamp_units.push_back(e - s);
                 ctx->ksamp_n++;
             } else {
-                k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits, nextCount,
-                                                           sink);
+                k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
         }
-        pool_put(ctx, sink);   // reused only by later work on the same stream
     }
     GLS_HIP(ctx, hipGetLastError());
     return GLS_OK;
