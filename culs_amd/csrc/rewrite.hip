@@ -104,65 +104,50 @@ __device__ __forceinline__ void unpack_cut(const uint32_t *p, DCut &c) {
     c.used = (m >> 30) & 1;
 }
 
-__device__ int find_cut(DCut *cs) {
-    for (int i = 0; i < NCUT; i++)
-        if (!cs[i].used) return i;
-    int ans = -1;
-    for (int i = 0; i < NCUT; i++)
-        if (cs[i].nLeaves > 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
-    if (ans == -1)
-        for (int i = 0; i < NCUT; i++)
-            if (cs[i].nLeaves == 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
-    if (ans == -1)
-        for (int i = 0; i < NCUT; i++)
-            if (cs[i].nLeaves < 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
-    cs[ans].used = 0;
-    return ans;
-}
-
-// sorted merge of two leaf sets, a.nLeaves >= b.nLeaves; fails above 4 leaves
-__device__ int merge_ordered(const DCut &a, const DCut &b, DCut &c) {
-    int n = 0, i = 0, k = 0;
-    for (n = 0; n < 4; n++) {
-        if (k == b.nLeaves) {
-            if (i == a.nLeaves) { c.nLeaves = n; return 1; }
-            c.leaves[n] = a.leaves[i++];
-            continue;
-        }
-        if (i == a.nLeaves) { c.leaves[n] = b.leaves[k++]; continue; }
-        if (a.leaves[i] < b.leaves[k]) { c.leaves[n] = a.leaves[i++]; continue; }
-        if (a.leaves[i] > b.leaves[k]) { c.leaves[n] = b.leaves[k++]; continue; }
-        c.leaves[n] = a.leaves[i++];
-        k++;
+// Sorted union of two sorted leaf sets (rewrite.cu:112-163); fails above 4
+// leaves.  Each leaf goes to its rank in the union, computed by compares, and
+// every output slot selects its leaf from the eight candidates, so the leaf
+// arrays are indexed by constants only and stay in registers.
+__device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut &c) {
+    bool nb[4];    // leaf j of b is new: valid and not a leaf of a
+    int ra[4], rb[4], n = a.nLeaves;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        bool d = false;
+#pragma unroll
+        for (int i = 0; i < 4; i++) d |= (i < a.nLeaves && a.leaves[i] == b.leaves[j]);
+        nb[j] = j < b.nLeaves && !d;
+        n += nb[j];
     }
-    if (i < a.nLeaves || k < b.nLeaves) return 0;
+    if (n > 4) return 0;
+    // rank in the union = smaller leaves of a + smaller new leaves of b
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int r = i;
+#pragma unroll
+        for (int j = 0; j < 4; j++) r += (nb[j] && b.leaves[j] < a.leaves[i]);
+        ra[i] = i < a.nLeaves ? r : 8;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int r = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r += (i < a.nLeaves && a.leaves[i] < b.leaves[j]);
+#pragma unroll
+        for (int k = 0; k < j; k++) r += nb[k];
+        rb[j] = nb[j] ? r : 8;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) x = ra[i] == p ? a.leaves[i] : x;
+#pragma unroll
+        for (int j = 0; j < 4; j++) x = rb[j] == p ? b.leaves[j] : x;
+        c.leaves[p] = x;
+    }
     c.nLeaves = n;
     return 1;
-}
-
-__device__ int dominates(const DCut &a, const DCut &b) {
-    for (int i = 0; i < a.nLeaves; i++) {
-        int ok = 0;
-        for (int j = 0; j < b.nLeaves; j++) ok |= (b.leaves[j] == a.leaves[i]);
-        if (!ok) return 0;
-    }
-    return 1;
-}
-
-__device__ int cut_filter(DCut *cs, int id) {
-    uint32_t sid = cut_sign(cs[id]);
-    for (int i = 0; i < NCUT; i++) {
-        if (i == id || !cs[i].used) continue;
-        uint32_t si = cut_sign(cs[i]);
-        if (cs[i].nLeaves > cs[id].nLeaves) {
-            if ((si & sid) != sid) continue;
-            if (dominates(cs[id], cs[i])) cs[i].used = 0;
-        } else {
-            if ((si & sid) != si) continue;
-            if (dominates(cs[i], cs[id])) { cs[id].used = 0; return 1; }
-        }
-    }
-    return 0;
 }
 
 __device__ __forceinline__ uint32_t swap_adj(uint32_t t, int v) {
@@ -182,11 +167,13 @@ __device__ uint32_t tt_stretch(uint32_t t, int nVar, int phase) {
     return t;
 }
 
-__device__ int truth_phase(const DCut &x, const DCut &c) {
+__device__ __forceinline__ int truth_phase(const DCut &x, const DCut &c) {
     int ph = 0;
-    for (int i = 0; i < c.nLeaves; i++)
-        for (int j = 0; j < x.nLeaves; j++)
-            if (x.leaves[j] == c.leaves[i]) ph |= 1 << i;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (i < c.nLeaves && j < x.nLeaves && x.leaves[j] == c.leaves[i]) ph |= 1 << i;
     return ph;
 }
 
@@ -200,7 +187,7 @@ __device__ uint32_t tt_shrink(uint32_t t, int phase) {
     return t;
 }
 
-__device__ int minimize_support(DCut &c) {
+__device__ __forceinline__ int minimize_support(DCut &c) {
     const uint32_t m0[4] = {0x5555, 0x3333, 0x0F0F, 0x00FF};
     const uint32_t m1[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
     int phase = 0, nL = c.nLeaves;
@@ -213,9 +200,20 @@ __device__ int minimize_support(DCut &c) {
     if (nL == c.nLeaves) return 0;
     t = tt_shrink(t, phase);
     c.tt = t & 0xFFFF;
-    int k = 0;
-    for (int i = 0; i < 4; i++)
-        if (i < c.nLeaves && ((phase >> i) & 1)) c.leaves[k++] = c.leaves[i];
+    // compaction: kept leaf i moves to slot popc(phase below i); the slots past
+    // the new count keep their old contents, as an in-place compaction would
+    uint32_t o[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) o[p] = c.leaves[p];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int pos = __popc((uint32_t)phase & ((1u << i) - 1u));
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            if (p == pos && ((phase >> i) & 1)) o[p] = c.leaves[i];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) c.leaves[p] = o[p];
     c.nLeaves = nL;
     return 1;
 }
@@ -363,7 +361,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     m.nLeaves = 0; m.tt = 0; m.value = 0; m.used = 0;
     mm = m;
     if (a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4) {
-        int ok = a.nLeaves >= b.nLeaves ? merge_ordered(a, b, m) : merge_ordered(b, a, m);
+        const int ok = merge_ordered(a, b, m);
         if (!ok) {
             status = C_FAIL;
         } else {
