@@ -82,12 +82,16 @@ __device__ __forceinline__ uint32_t cut_sign(const DCut &c) {
 }
 
 __device__ __forceinline__ uint32_t cut_value(const DCut &c, const int32_t *nRef) {
+    // all four loads in flight at once; absent leaves read node 0's entry
+    int r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] = nRef[i < c.nLeaves ? c.leaves[i] : 0u];
     int value = 0, nOnes = 0;
+#pragma unroll
     for (int i = 0; i < 4; i++)
         if (i < c.nLeaves) {
-            int r = nRef[c.leaves[i]];
-            value += r;
-            nOnes += (r == 1);
+            value += r[i];
+            nOnes += (r[i] == 1);
         }
     if (c.nLeaves < 2) return 1001;
     if (value > 1000) value = 1000;
@@ -254,6 +258,17 @@ __device__ __forceinline__ bool leaf_in(uint32_t x, uint32_t s0, uint32_t s1, ui
     return (n > 0 && x == s0) || (n > 1 && x == s1) || (n > 2 && x == s2) || (n > 3 && x == s3);
 }
 
+// Minimum of v over lanes 0..7, valid in lane 0: two quad_perm steps (xor 1,
+// xor 2) and row rotations by 4 and 12, so lane 0 also sees lanes 4..7 in
+// either rotation direction.  DPP moves, no LDS round trip.
+__device__ __forceinline__ uint32_t min8_lane0(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0xB1, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0x4E, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0x124, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0x12C, 0xF, 0xF, false));
+    return v;
+}
+
 struct Slot {
     uint32_t l0, l1, l2, l3, tt, value;
     int n, used;
@@ -394,17 +409,11 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         if (usedMask != 0xFFull) {
             id = __ffsll((unsigned long long)(~usedMask & 0xFFull)) - 1;
         } else {
-            id = -1;
-            for (int cat = 0; cat < 3 && id < 0; cat++) {
-                const bool in = lane < NCUT && (cat == 0 ? s.n > 2 : cat == 1 ? s.n == 2 : s.n < 2);
-                uint32_t key = in ? ((s.value << 3) | (uint32_t)lane) : 0xFFFFFFFFu;
-                for (int off = 1; off < NCUT; off <<= 1) {
-                    uint32_t o = (uint32_t)__shfl_xor((int)key, off);
-                    key = o < key ? o : key;
-                }
-                key = (uint32_t)rl((int)key, 0);
-                if (key != 0xFFFFFFFFu) id = (int)(key & 7u);
-            }
+            // categories in order (more than 2 leaves, 2, fewer), then the
+            // smallest value, then the first slot: one minimum of that key
+            const uint32_t cat = s.n > 2 ? 0u : s.n == 2 ? 1u : 2u;
+            const uint32_t key = lane < NCUT ? (cat << 14 | (s.value & 0x7FF) << 3 | (uint32_t)lane) : 0xFFFFFFFFu;
+            id = (int)((uint32_t)rl((int)min8_lane0(key), 0) & 7u);
         }
         if (lane == id) s.used = 0;
         if (st == C_FAIL) continue;

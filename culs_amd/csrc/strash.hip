@@ -78,15 +78,25 @@ __global__ void k_kahn_count(int32_t n, int32_t nPIs, const uint32_t *__restrict
     }
 }
 
+// Fanout CSR: adj holds the fanout, adjY its other fanin (the fanin that is
+// not the list's owner).
 __global__ void k_kahn_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                             const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
-                            uint32_t *cursor, uint32_t *adj) {
+                            uint32_t *cursor, uint32_t *adj, uint32_t *adjY) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
         if (dead[v]) continue;
         uint32_t a = LID(f0[v]), b = LID(f1[v]);
-        if (a > (uint32_t)nPIs) adj[atomicAdd(&cursor[a], 1u)] = (uint32_t)v;
-        if (b > (uint32_t)nPIs) adj[atomicAdd(&cursor[b], 1u)] = (uint32_t)v;
+        if (a > (uint32_t)nPIs) {
+            const uint32_t e = atomicAdd(&cursor[a], 1u);
+            adj[e] = (uint32_t)v;
+            adjY[e] = b;
+        }
+        if (b > (uint32_t)nPIs) {
+            const uint32_t e = atomicAdd(&cursor[b], 1u);
+            adj[e] = (uint32_t)v;
+            adjY[e] = a;
+        }
     }
 }
 
@@ -264,99 +274,305 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
     }
 }
 
-// Consecutive narrow waves inside one workgroup.  Waves are separated by
-// workgroup barriers instead of kernel boundaries, the frontier lives in LDS,
-// atomics are workgroup-scoped (performed in the local L2), and each wave's
-// representatives are resolved right after its inserts, so the next wave maps
-// a fanin with a single load.  A node entering the next frontier has its fanin
-// literals and fanout range fetched into LDS at once, by the thread that
-// released it: the next wave then starts with the representative reads and the
-// fanout-counter atomics instead of first fetching them.  Runs while the
-// frontier is non-empty and at most `thresh` (<= FRONT_MAX) wide; *kOut = next
-// wave to run.
+// Packed per-node record for the in-workgroup sweep: fanin literals, the
+// pre-inserted table slot, the fanout count, where the fanout list starts in
+// adj, then the first REC_PAIRS fanouts, each followed by its other fanin.
+// One 128-byte line per node: a node entering the frontier reads the 32-byte
+// header and only the 16-byte pair chunks its fanout count needs.
+constexpr int REC_PAIRS = 12;
+struct alignas(128) NodeRec {
+    uint32_t f0, f1, slot, cnt, adj, pad0, pad1, pad2;
+    uint32_t wy[2 * REC_PAIRS];     // fanout j: wy[2j], its other fanin: wy[2j + 1]
+};
+static_assert(sizeof(NodeRec) == 128 && REC_PAIRS == 12, "one cache line per record, 6 pair chunks");
+
+__global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
+                            const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
+                            const uint32_t *__restrict__ slotOf, const uint32_t *__restrict__ adjOff,
+                            const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY, NodeRec *recs) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (dead[v]) continue;
+        NodeRec r;
+        r.f0 = f0[v];
+        r.f1 = f1[v];
+        r.slot = slotOf[v];
+        r.adj = adjOff[v];
+        r.cnt = adjOff[v + 1] - r.adj;
+        r.pad0 = r.pad1 = r.pad2 = 0;
+#pragma unroll
+        for (int j = 0; j < REC_PAIRS; j++) {
+            r.wy[2 * j] = (uint32_t)j < r.cnt ? adj[r.adj + j] : 0u;
+            r.wy[2 * j + 1] = (uint32_t)j < r.cnt ? adjY[r.adj + j] : 0u;
+        }
+        recs[v] = r;
+    }
+}
+
+// Header of a record and its first pair chunk (fanouts 0 and 1).
+struct RecHead {
+    uint32_t f0, f1, slot, cnt, adj;
+    uint4 c0;
+};
+
+__device__ __forceinline__ RecHead load_head(const NodeRec *recs, uint32_t v) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
+    const uint4 h0 = p[0], h1 = p[1];
+    RecHead h;
+    h.f0 = h0.x; h.f1 = h0.y; h.slot = h0.z; h.cnt = h0.w; h.adj = h1.x;
+    h.c0 = p[2];
+    return h;
+}
+
+// Appends node v's fanout edges (fanout, other fanin, source entry q) at
+// position eb of an LDS edge list: chunk 0 comes with the header, chunks 1..5
+// are read as the count needs them, fanouts past REC_PAIRS from adj / adjY.
+__device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecHead &h,
+                                          const NodeRec *recs, uint32_t v, uint16_t q,
+                                          const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
+    uint4 c[6];
+    c[0] = h.c0;
+#pragma unroll
+    for (int k = 1; k < 6; k++) c[k] = h.cnt > 2u * k ? p[2 + k] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        if (h.cnt > 2u * k) { gw[eb + 2 * k] = c[k].x; gy[eb + 2 * k] = c[k].y; gq[eb + 2 * k] = q; }
+        if (h.cnt > 2u * k + 1) { gw[eb + 2 * k + 1] = c[k].z; gy[eb + 2 * k + 1] = c[k].w; gq[eb + 2 * k + 1] = q; }
+    }
+    for (uint32_t j = REC_PAIRS; j < h.cnt; j++) {
+        gw[eb + j] = adj[h.adj + j];
+        gy[eb + j] = adjY[h.adj + j];
+        gq[eb + j] = q;
+    }
+}
+
+// Consecutive narrow waves inside one workgroup, with every hand-over between
+// waves kept in LDS so that a wave costs about one dependent trip to memory:
+//   - the frontier entries (node, fanin literals, pre-inserted slot and where
+//     each fanin's representative comes from) and the frontier's fanout edges
+//     (fanout, its other fanin, source entry) are in LDS, written by the
+//     thread that released the node;
+//   - phase A: one thread per node hashes it (fanin representatives from LDS,
+//     one returning atomic on the table), and one thread per fanout edge
+//     decrements the fanout's pending count while loading its NodeRec and the
+//     representative of its other fanin, all at once; a fanout that becomes
+//     ready enters the next frontier (its other fanin is resolved, or is in
+//     this wave, whose frontier position its rl then holds);
+//   - phase B: each node's representative is the table minimum: a value from
+//     an earlier wave, returned by its own atomic, is final; otherwise it is the
+//     smallest id among this wave's nodes on the same slot, kept in an LDS table
+//     whose keys carry the wave number (stale entries count as empty).
+// rl of a node that has entered the frontier and is not hashed yet holds
+// INVALID32 | position << 32.  Runs while the frontier fits (at most `thresh`
+// <= WL_FM nodes, at most WL_EM edges); *kOut = next wave to run (== k0 when
+// even the first frontier's edges do not fit).
 constexpr int WG1 = 1024;
-constexpr int FRONT_MAX = 2048;
-constexpr int WG1_ITEMS = FRONT_MAX / WG1;
-__global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh, WaveArgs a, uint32_t *kOut) {
-    __shared__ uint32_t s_front[2][FRONT_MAX];
-    __shared__ uint32_t s_f0[2][FRONT_MAX], s_f1[2][FRONT_MAX];      // fanin literals
-    __shared__ uint32_t s_e0[2][FRONT_MAX], s_e1[2][FRONT_MAX];      // fanout range in adj
-    __shared__ uint32_t s_next;
+constexpr int WL_FM = 1024;     // frontier entries
+constexpr int WL_EM = 2048;     // fanout edges of a frontier
+constexpr int WL_DT_BITS = 11;  // same-wave duplicate table: 2048 entries
+constexpr int WL_DT = 1 << WL_DT_BITS;
+constexpr unsigned long long SRC_POS = 1ull << 63;
+static_assert(WL_FM <= WG1 && WL_EM <= 2 * WG1, "one node and two edges per thread");
+static_assert(WL_FM <= 65536, "entry index in 16 bits");
+
+// Source of a fanin's representative: SRC_POS | frontier position of this
+// wave, or the resolved rl value (literal | level << 32).
+__device__ __forceinline__ unsigned long long src_of(unsigned long long r) {
+    return (uint32_t)r == INVALID32 ? (SRC_POS | (r >> 32)) : r;
+}
+
+__global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh, WaveArgs a,
+                                                   const NodeRec *__restrict__ recs,
+                                                   const uint32_t *__restrict__ adjY, uint32_t *kOut) {
+    __shared__ uint32_t e_v[2][WL_FM], e_f0[2][WL_FM], e_f1[2][WL_FM], e_slot[2][WL_FM];
+    __shared__ unsigned long long e_s0[2][WL_FM], e_s1[2][WL_FM];
+    __shared__ uint32_t g_w[2][WL_EM], g_y[2][WL_EM];
+    __shared__ uint16_t g_q[2][WL_EM];
+    __shared__ unsigned long long s_rep[WL_FM];
+    __shared__ unsigned long long d_key[WL_DT], d_val[WL_DT];
+    __shared__ uint32_t s_cnt[2], s_ecnt[2], s_over;
     const uint32_t tid = threadIdx.x;
+    const int32_t nPIs = a.nPIs;
     uint32_t k = k0;
     const uint32_t pn = a.fcnt[k - 1], ps = a.fstart[k - 1];
     uint32_t cn = a.fcnt[k], cs = a.fstart[k];
+    for (uint32_t i = tid; i < WL_DT; i += WG1) { d_key[i] = 0; d_val[i] = 0; }
+    if (tid == 0) { s_cnt[0] = 0; s_cnt[1] = 0; s_ecnt[0] = 0; s_ecnt[1] = 0; s_over = 0; }
     // the previous wave may come from the grid-wide kernel, unresolved
     for (uint32_t i = tid; i < pn; i += WG1) resolve<__HIP_MEMORY_SCOPE_WORKGROUP>(a, a.order[ps + i]);
-    if (cn <= thresh)
-        for (uint32_t i = tid; i < cn; i += WG1) {
-            const uint32_t v = a.order[cs + i];
-            s_front[0][i] = v;
-            s_f0[0][i] = a.f0[v];
-            s_f1[0][i] = a.f1[v];
-            s_e0[0][i] = a.adjOff[v];
-            s_e1[0][i] = a.adjOff[v + 1];
-        }
     __syncthreads();
-    int buf = 0;
+    if (cn > thresh || cn > (uint32_t)WL_FM) {
+        if (tid == 0) *kOut = k;
+        return;
+    }
+    // first frontier from global memory: mark positions, then load records
+    for (uint32_t i = tid; i < cn; i += WG1) {
+        const uint32_t v = a.order[cs + i];
+        e_v[0][i] = v;
+        a.rl[v] = (unsigned long long)INVALID32 | (unsigned long long)i << 32;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cn; i += WG1) {
+        const uint32_t v = e_v[0][i];
+        const RecHead r = load_head(recs, v);
+        const uint32_t u0 = LID(r.f0), u1 = LID(r.f1);
+        e_f0[0][i] = r.f0;
+        e_f1[0][i] = r.f1;
+        e_slot[0][i] = r.slot;
+        // fanins are in earlier waves, all resolved
+        e_s0[0][i] = u0 <= (uint32_t)nPIs ? 2ull * u0 : a.rl[u0];
+        e_s1[0][i] = u1 <= (uint32_t)nPIs ? 2ull * u1 : a.rl[u1];
+        const uint32_t eb = atomicAdd(&s_ecnt[0], r.cnt);
+        if (eb + r.cnt <= (uint32_t)WL_EM) put_edges(g_w[0], g_y[0], g_q[0], eb, r, recs, v, (uint16_t)i, a.adj, adjY);
+        else s_over = 1;
+    }
+    __syncthreads();
+    if (s_over) {
+        if (tid == 0) *kOut = k;
+        return;
+    }
+    int b = 0;
     uint32_t last = 0;
-    while (cn > 0 && cn <= thresh) {
-        const uint32_t nb = cs + cn;
-        if (tid == 0) s_next = 0;
-        __syncthreads();
-        // tasks [0, cn) hash node i, tasks [cn, 2cn) release the fanouts of
-        // node i - cn: both run at once on different threads
-        uint32_t myV[WG1_ITEMS], mySlot[WG1_ITEMS], myLev[WG1_ITEMS];
-        bool myNT[WG1_ITEMS];
-#pragma unroll
-        for (int j = 0; j < WG1_ITEMS; j++) {
-            const uint32_t i = tid + (uint32_t)j * WG1;
-            myNT[j] = false;
-            myV[j] = 0;
-            if (i < cn) {
-                const uint32_t v = s_front[buf][i];
-                myV[j] = v;
-                myNT[j] = hash_node_lits<__HIP_MEMORY_SCOPE_WORKGROUP>(a, k, v, s_f0[buf][i], s_f1[buf][i],
-                                                                         mySlot[j], myLev[j]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 2 * WG1_ITEMS; j++) {
-            const uint32_t i = tid + (uint32_t)j * WG1;
-            if (i >= cn && i < 2 * cn) {
-                const uint32_t q = i - cn;
-                release_range<__HIP_MEMORY_SCOPE_WORKGROUP>(a, s_e0[buf][q], s_e1[buf][q], [&](uint32_t w) {
-                    const uint32_t p = atomicAdd(&s_next, 1u);
-                    a.order[nb + p] = w;
-                    if (p < FRONT_MAX) {
-                        s_front[buf ^ 1][p] = w;
-                        s_f0[buf ^ 1][p] = a.f0[w];
-                        s_f1[buf ^ 1][p] = a.f1[w];
-                        s_e0[buf ^ 1][p] = a.adjOff[w];
-                        s_e1[buf ^ 1][p] = a.adjOff[w + 1];
+    while (true) {
+        const int nb = b ^ 1;
+        const uint32_t ne = s_ecnt[b], nbase = cs + cn;
+        // ---- phase A: hash this wave's nodes
+        bool have = false, tri = false, fin = false;
+        uint32_t v = 0, r = 0, lev = 0, dh = 0;
+        if (tid < cn) {
+            have = true;
+            v = e_v[b][tid];
+            const uint32_t raw0 = e_f0[b][tid], raw1 = e_f1[b][tid];
+            const unsigned long long s0 = e_s0[b][tid], s1 = e_s1[b][tid];
+            const unsigned long long x0 = (s0 & SRC_POS) ? s_rep[(uint32_t)s0] : s0;
+            const unsigned long long x1 = (s1 & SRC_POS) ? s_rep[(uint32_t)s1] : s1;
+            uint32_t l0 = (uint32_t)x0 ^ LC(raw0), l1 = (uint32_t)x1 ^ LC(raw1);
+            uint32_t la = (uint32_t)(x0 >> 32), lb = (uint32_t)(x1 >> 32);
+            if (l0 > l1) { uint32_t t = l0; l0 = l1; l1 = t; t = la; la = lb; lb = t; }
+            r = INVALID32;
+            if (l0 == l1) { r = l0; lev = la; }
+            else if (l0 == (l1 ^ 1u)) { r = 0; lev = 0; }
+            else if (l0 == 1u) { r = l1; lev = lb; }
+            else if (l0 == 0u) { r = 0; lev = 0; }
+            tri = r != INVALID32;
+            if (!tri) {
+                lev = 1u + (la > lb ? la : lb);
+                const unsigned long long val = (static_cast<unsigned long long>(k) << 32) | v;
+                const uint32_t r0 = raw0 < raw1 ? raw0 : raw1, r1 = raw0 < raw1 ? raw1 : raw0;
+                uint32_t slot = e_slot[b][tid];
+                if (!(l0 == r0 && l1 == r1 && slot != NO_SLOT)) {
+                    // fanins changed: probe for the mapped key
+                    const uint64_t key = and_key(l0, l1);
+                    uint64_t s = mix64(key) & a.mask;
+                    slot = NO_SLOT;
+                    for (uint64_t probe = 0; probe <= a.mask; ++probe) {
+                        unsigned long long prev = KEY_EMPTY;
+                        __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key,
+                                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
+                        s = (s + 1) & a.mask;
                     }
-                });
+                    if (slot == NO_SLOT) { atomicOr(a.err, 1u); slot = 0; }
+                    a.slotOf[v] = slot;
+                }
+                const unsigned long long prev = __hip_atomic_fetch_min(&a.ht[2 * (size_t)slot + 1], val,
+                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (prev != KEY_EMPTY && (uint32_t)(prev >> 32) < k) {
+                    fin = true;                    // an earlier wave holds the slot
+                    r = 2u * (uint32_t)prev;
+                } else {
+                    // smallest id of this wave on the slot
+                    const unsigned long long dkey = (static_cast<unsigned long long>(k) << 32) | slot;
+                    uint32_t h = (slot * 0x9E3779B1u) >> (32 - WL_DT_BITS);
+                    for (int probe = 0; probe < WL_DT; probe++) {
+                        unsigned long long cur = __hip_atomic_load(&d_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if ((uint32_t)(cur >> 32) != k) {
+                            unsigned long long exp = cur;
+                            __hip_atomic_compare_exchange_strong(&d_key[h], &exp, dkey, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            cur = exp == cur ? dkey : exp;
+                        }
+                        if (cur == dkey) break;
+                        h = (h + 1) & (WL_DT - 1);
+                    }
+                    atomicMax(&d_val[h], (static_cast<unsigned long long>(k) << 32) | (uint32_t)~v);
+                    dh = h;
+                }
             }
         }
+        // ---- phase A: release the fanout edges
+        for (uint32_t t = tid; t < ne; t += WG1) {
+            const uint32_t w = g_w[b][t];
+            const uint32_t left = __hip_atomic_fetch_add(&a.pending[w], 0xFFFFFFFFu, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (left != 1u) continue;
+            // w is ready: its record header and its other fanin's representative
+            const uint32_t y = g_y[b][t], q = g_q[b][t];
+            const uint32_t x = e_v[b][q];
+            const RecHead rw = load_head(recs, w);
+            const bool yLoad = y != x && y > (uint32_t)nPIs;
+            const unsigned long long ry = yLoad ? a.rl[y] : 0ull;
+            const uint32_t p = atomicAdd(&s_cnt[nb], 1u);
+            a.order[nbase + p] = w;
+            if (p >= (uint32_t)WL_FM) { s_over = 1; continue; }
+            const unsigned long long sx = SRC_POS | q;
+            const unsigned long long sy = y == x ? sx : y <= (uint32_t)nPIs ? 2ull * y : src_of(ry);
+            e_v[nb][p] = w;
+            e_f0[nb][p] = rw.f0;
+            e_f1[nb][p] = rw.f1;
+            e_slot[nb][p] = rw.slot;
+            e_s0[nb][p] = LID(rw.f0) == x ? sx : sy;
+            e_s1[nb][p] = LID(rw.f1) == x ? sx : sy;
+            a.rl[w] = (unsigned long long)INVALID32 | (unsigned long long)p << 32;
+            const uint32_t eb = atomicAdd(&s_ecnt[nb], rw.cnt);
+            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges(g_w[nb], g_y[nb], g_q[nb], eb, rw, recs, w, (uint16_t)p, a.adj, adjY);
+            else s_over = 1;
+        }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < WG1_ITEMS; j++)
-            if (myNT[j]) a.rl[myV[j]] = pack_rl(rep_of_slot<__HIP_MEMORY_SCOPE_WORKGROUP>(a, mySlot[j]), myLev[j]);
-        const uint32_t nn = s_next;
+        // ---- phase B: representatives of this wave
+        if (have) {
+            if (!tri && !fin) r = 2u * ~(uint32_t)d_val[dh];
+            const unsigned long long x = pack_rl(r, lev);
+            s_rep[tid] = x;
+            a.rl[v] = x;
+        }
+        const uint32_t nn = s_cnt[nb];
+        const bool over = s_over != 0;
         if (tid == 0) {
-            a.fstart[k + 1] = nb;
+            a.fstart[k + 1] = nbase;
             a.fcnt[k + 1] = nn;
+            s_cnt[b] = 0;
+            s_ecnt[b] = 0;
         }
         last = k;
-        cs = nb;
-        cn = nn;
         ++k;
-        buf ^= 1;
+        cs = nbase;
+        cn = nn;
+        b = nb;
         __syncthreads();
+        if (over || cn == 0 || cn > thresh) break;
     }
     if (tid == 0) {
         *kOut = k;
         if (last) atomicMax(a.depth, last);
+    }
+}
+
+// wave of every processed node from the per-wave ranges of order[]: wave k
+// holds positions [fstart[k], fstart[k + 1]) for k = 1..depth
+__global__ void k_wave_of_order(uint32_t processed, uint32_t depth, const uint32_t *__restrict__ fstart,
+                                const uint32_t *__restrict__ order, uint32_t *wave) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < processed; i += stride) {
+        uint32_t lo = 1, hi = depth;     // largest k with fstart[k] <= i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (fstart[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        wave[order[i]] = lo;
     }
 }
 
@@ -484,7 +700,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     Tmp t{ctx};
     uint32_t *ref = t.get<uint32_t>(n), *order = t.get<uint32_t>(n), *pending = t.get<uint32_t>(n);
     uint32_t *cnt = t.get<uint32_t>(n + 1), *adjOff = t.get<uint32_t>(n + 1), *cursor = t.get<uint32_t>(n + 1);
-    uint32_t *adj = t.get<uint32_t>(2 * (size_t)n + 1);
+    uint32_t *adj = t.get<uint32_t>(2 * (size_t)n + 1), *adjY = t.get<uint32_t>(2 * (size_t)n + 1);
     unsigned long long *rl = t.get<unsigned long long>(n);
     uint32_t *wave = t.get<uint32_t>(n), *newId = t.get<uint32_t>(n);
     uint32_t *survIds = t.get<uint32_t>(n), *survWave = t.get<uint32_t>(n);
@@ -492,6 +708,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     uint8_t *dead = t.get<uint8_t>(n), *surv = t.get<uint8_t>(n);
     uint32_t *slotOf = t.get<uint32_t>(n), *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
     uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
+    NodeRec *recs = t.get<NodeRec>(n);
     SweepCtl *ctl = t.get<SweepCtl>(1);
     // room for every live node's own key plus a changed key per node
     uint64_t cap = 1024;
@@ -499,7 +716,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     unsigned long long *ht = t.get<unsigned long long>(2 * cap);
     if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
         !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !slotOf ||
-        !fcnt || !fstart || !d_misc || !ctl || !ht)
+        !fcnt || !fstart || !d_misc || !ctl || !ht || !recs || !adjY)
         return gls_fail(ctx, GLS_ERR_NOMEM, "strash: device allocation failed (n=%d)", n);
 
     GLS_HIP(ctx, hipMemsetAsync(ref, 0, sizeof(uint32_t) * n, st));
@@ -537,25 +754,34 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: scan storage");
     GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
-    k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj);
+    k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj, adjY);
     k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
     k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotOf, &d_misc[1]);
+    k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, slotOf, adjOff, adj, adjY, recs);
     GLS_HIP(ctx, hipGetLastError());
 
     // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
     //    ones as one grid-wide kernel each
     WaveArgs a{in.f0, in.f1, nPIs, rl, slotOf, wave, ht, cap - 1, adjOff, adj, pending, order,
                fcnt, fstart, &d_misc[0], &d_misc[1]};
-    const uint32_t kThresh = FRONT_MAX;
+    const uint32_t kThresh = WL_FM;
     uint32_t k = 1, waves = 0, cur = 0;
     for (;;) {
         GLS_TRY(stream_copy(ctx, &cur, &fcnt[k], sizeof(uint32_t), hipMemcpyDeviceToHost));
         if (cur == 0) break;
         if (cur <= kThresh) {
-            k_waves_1wg<<<1, WG1, 0, st>>>(k, kThresh, a, &d_misc[2]);
+            k_waves_1wg<<<1, WG1, 0, st>>>(k, kThresh, a, recs, adjY, &d_misc[2]);
             GLS_HIP(ctx, hipGetLastError());
             uint32_t kn = k;
             GLS_TRY(stream_copy(ctx, &kn, &d_misc[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (kn == k) {
+                // the frontier's fanout edges do not fit the workgroup: one grid-wide wave
+                k_wave_multi<<<sweep_grid(cur), kBlock, 0, st>>>(k, a);
+                GLS_HIP(ctx, hipGetLastError());
+                ++k;
+                ++waves;
+                continue;
+            }
             waves += kn - k;
             k = kn;
         } else {
@@ -575,6 +801,8 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
         return gls_fail(ctx, GLS_ERR_ARG, "strash: %u of %u AND nodes unreachable in level order (cycle)",
                         nAnds - nDead - processed, nAnds - nDead);
 
+    if (processed > 0)
+        k_wave_of_order<<<grid_for(processed), kBlock, 0, st>>>(processed, (uint32_t)depth, fstart, order, wave);
     if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(n, nPIs, wave, tagIn, tagDepth);
     // 4. survivors in (wave, id) order
     k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, slotOf, ht, surv);
