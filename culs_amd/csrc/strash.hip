@@ -101,20 +101,26 @@ __global__ void k_kahn_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict_
 }
 
 // Pre-inserts the key of every live non-trivial AND as read from its own
-// fanin literals, leaving the value empty; slotOf[v] = its slot (NO_SLOT for a
-// trivial key).  A node whose fanins turn out to be their own representatives
-// keeps this key, so in the sweep it only lowers the slot's value and skips
-// the probe.
+// fanin literals; the slot's value field collects the smallest id with that
+// key; slotTmp[v] = its slot (NO_SLOT for a trivial key).  k_canon then gives
+// every such node the value location of its key: the entry val[m] of that
+// smallest id m, so that the nodes of one wave, whose ids are close, update
+// values on a few cache lines instead of one random table line each.  The
+// table keeps the key (for probes by nodes whose fanins changed) with the
+// value field set to VAL_MARK | m, which sends those probes to val[m] too.
 constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
+constexpr uint32_t LOC_HT = 0x80000000u;                 // location is a table slot
+constexpr unsigned long long VAL_MARK = 0xFFFFFFFE00000000ull;
+
 __global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
-                            const uint8_t *__restrict__ dead, unsigned long long *ht, uint64_t mask, uint32_t *slotOf,
+                            const uint8_t *__restrict__ dead, unsigned long long *ht, uint64_t mask, uint32_t *slotTmp,
                             uint32_t *err) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
         if (dead[v]) continue;
         uint32_t l0 = f0[v], l1 = f1[v];
         if (l0 > l1) { const uint32_t t = l0; l0 = l1; l1 = t; }
-        if (l0 == l1 || l0 == (l1 ^ 1u) || l0 < 2u) { slotOf[v] = NO_SLOT; continue; }
+        if (l0 == l1 || l0 == (l1 ^ 1u) || l0 < 2u) { slotTmp[v] = NO_SLOT; continue; }
         const uint64_t key = and_key(l0, l1);
         uint64_t s = mix64(key) & mask;
         uint32_t slot = NO_SLOT;
@@ -125,8 +131,25 @@ __global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict_
             if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
             s = (s + 1) & mask;
         }
-        if (slot == NO_SLOT) atomicOr(err, 1u);
-        slotOf[v] = slot;
+        if (slot == NO_SLOT) { atomicOr(err, 1u); continue; }
+        atomicMin(&ht[2 * (size_t)slot + 1], (unsigned long long)v);
+        slotTmp[v] = slot;
+    }
+}
+
+__global__ void k_canon(int32_t n, int32_t nPIs, const uint8_t *__restrict__ dead, unsigned long long *ht,
+                        const uint32_t *__restrict__ slotTmp, uint32_t *locOf) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (dead[v]) continue;
+        const uint32_t slot = slotTmp[v];
+        if (slot == NO_SLOT) { locOf[v] = NO_SLOT; continue; }
+        // the low word is the smallest id before and after the owner marks it
+        const uint32_t m = (uint32_t)__hip_atomic_load(&ht[2 * (size_t)slot + 1], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        locOf[v] = m;
+        if (m == (uint32_t)v)
+            __hip_atomic_store(&ht[2 * (size_t)slot + 1], VAL_MARK | m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -135,8 +158,9 @@ struct WaveArgs {
     int32_t nPIs;
     unsigned long long *rl;         // per node: literal over representative ids | level << 32;
                                     // the literal is INVALID32 until the representative is known
-    uint32_t *slotOf, *wave;        // table slot of non-trivial nodes, wave of every live node
-    unsigned long long *ht;         // slot s: ht[2s] = key, ht[2s + 1] = min (wave << 32 | id)
+    uint32_t *locOf, *wave;         // value location of non-trivial nodes, wave of every live node
+    unsigned long long *ht;         // slot s: ht[2s] = key, ht[2s + 1] = value or VAL_MARK | val index
+    unsigned long long *val;        // per-node values of pre-inserted keys: min (wave << 32 | id)
     uint64_t mask;
     const uint32_t *adjOff, *adj;
     uint32_t *pending, *order;
@@ -148,10 +172,22 @@ __device__ __forceinline__ unsigned long long pack_rl(uint32_t lit, uint32_t lev
     return (static_cast<unsigned long long>(lev) << 32) | lit;
 }
 
+// Value of a location: val[loc], or the table slot's value field for LOC_HT.
+__device__ __forceinline__ unsigned long long *vptr(const WaveArgs &a, uint32_t loc) {
+    return (loc & LOC_HT) ? &a.ht[2 * (size_t)(loc & ~LOC_HT) + 1] : &a.val[loc];
+}
+
+// Location of a key found or inserted by a probe at slot s.
 template <int SCOPE>
-__device__ __forceinline__ uint32_t rep_of_slot(const WaveArgs &a, uint32_t slot) {
-    unsigned long long val = __hip_atomic_load(&a.ht[2 * (size_t)slot + 1], __ATOMIC_RELAXED, SCOPE);
-    return 2u * (uint32_t)(val & 0xFFFFFFFFull);
+__device__ __forceinline__ uint32_t loc_of_slot(const WaveArgs &a, uint64_t s) {
+    const unsigned long long w = __hip_atomic_load(&a.ht[2 * s + 1], __ATOMIC_RELAXED, SCOPE);
+    return (w & 0xFFFFFFFF00000000ull) == VAL_MARK ? (uint32_t)w : ((uint32_t)s | LOC_HT);
+}
+
+template <int SCOPE>
+__device__ __forceinline__ uint32_t rep_of_loc(const WaveArgs &a, uint32_t loc) {
+    unsigned long long v = __hip_atomic_load(vptr(a, loc), __ATOMIC_RELAXED, SCOPE);
+    return 2u * (uint32_t)(v & 0xFFFFFFFFull);
 }
 
 // Fanin literal over representatives, and its level.  A fanin of the previous
@@ -163,7 +199,7 @@ __device__ __forceinline__ uint32_t map_fanin(const WaveArgs &a, uint32_t lit, u
     const unsigned long long x = a.rl[u];
     uint32_t r = (uint32_t)x;
     lev = (uint32_t)(x >> 32);
-    if (r == INVALID32) r = rep_of_slot<SCOPE>(a, a.slotOf[u]);
+    if (r == INVALID32) r = rep_of_loc<SCOPE>(a, a.locOf[u]);
     return r ^ LC(lit);
 }
 
@@ -192,29 +228,29 @@ __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, ui
     {
         // fanins that are their own representatives: the pre-inserted key holds
         const uint32_t r0 = raw0 < raw1 ? raw0 : raw1, r1 = raw0 < raw1 ? raw1 : raw0;
-        const uint32_t pre = a.slotOf[v];
+        const uint32_t pre = a.locOf[v];
         if (l0 == r0 && l1 == r1 && pre != NO_SLOT) {
-            __hip_atomic_fetch_min(&a.ht[2 * (size_t)pre + 1], (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
+            __hip_atomic_fetch_min(vptr(a, pre), (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
             slotOut = pre;
             return true;
         }
     }
     uint64_t s = mix64(key) & a.mask;
-    int64_t slot = -1;
+    uint32_t loc = NO_SLOT;
     for (uint64_t probe = 0; probe <= a.mask; ++probe) {
         unsigned long long prev = KEY_EMPTY;
         __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
                                              __ATOMIC_RELAXED, SCOPE);
         if (prev == KEY_EMPTY || prev == key) {
-            __hip_atomic_fetch_min(&a.ht[2 * s + 1], (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
-            slot = (int64_t)s;
+            loc = loc_of_slot<SCOPE>(a, s);
+            __hip_atomic_fetch_min(vptr(a, loc), (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
             break;
         }
         s = (s + 1) & a.mask;
     }
-    if (slot < 0) atomicOr(a.err, 1u);
-    slotOut = slot < 0 ? 0u : (uint32_t)slot;
-    a.slotOf[v] = slotOut;
+    if (loc == NO_SLOT) { atomicOr(a.err, 1u); loc = LOC_HT; }
+    slotOut = loc;
+    a.locOf[v] = loc;
     return true;
 }
 
@@ -249,7 +285,7 @@ __device__ __forceinline__ void release(const WaveArgs &a, uint32_t v, Emit emit
 template <int SCOPE>
 __device__ __forceinline__ void resolve(const WaveArgs &a, uint32_t v) {
     const unsigned long long x = a.rl[v];
-    if ((uint32_t)x == INVALID32) a.rl[v] = pack_rl(rep_of_slot<SCOPE>(a, a.slotOf[v]), (uint32_t)(x >> 32));
+    if ((uint32_t)x == INVALID32) a.rl[v] = pack_rl(rep_of_loc<SCOPE>(a, a.locOf[v]), (uint32_t)(x >> 32));
 }
 
 // One wave across the whole device (wide frontiers): items below pn resolve
@@ -288,7 +324,7 @@ static_assert(sizeof(NodeRec) == 128 && REC_PAIRS == 12, "one cache line per rec
 
 __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                             const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
-                            const uint32_t *__restrict__ slotOf, const uint32_t *__restrict__ adjOff,
+                            const uint32_t *__restrict__ locOf, const uint32_t *__restrict__ adjOff,
                             const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY, NodeRec *recs) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
@@ -296,7 +332,7 @@ __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict_
         NodeRec r;
         r.f0 = f0[v];
         r.f1 = f1[v];
-        r.slot = slotOf[v];
+        r.slot = locOf[v];
         r.adj = adjOff[v];
         r.cnt = adjOff[v + 1] - r.adj;
         r.pad0 = r.pad1 = r.pad2 = 0;
@@ -324,6 +360,38 @@ __device__ __forceinline__ RecHead load_head(const NodeRec *recs, uint32_t v) {
     return h;
 }
 
+// A whole record: header and all six pair chunks, one trip.
+struct RecFull {
+    uint32_t f0, f1, slot, cnt, adj;
+    uint4 c[6];
+};
+
+__device__ __forceinline__ RecFull load_full(const NodeRec *recs, uint32_t v) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
+    const uint4 h0 = p[0], h1 = p[1];
+    RecFull r;
+    r.f0 = h0.x; r.f1 = h0.y; r.slot = h0.z; r.cnt = h0.w; r.adj = h1.x;
+#pragma unroll
+    for (int k = 0; k < 6; k++) r.c[k] = p[2 + k];
+    return r;
+}
+
+// Appends the fanout edges of a node whose whole record is in registers.
+__device__ __forceinline__ void put_edges_full(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecFull &r,
+                                               uint16_t q, const uint32_t *__restrict__ adj,
+                                               const uint32_t *__restrict__ adjY) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        if (r.cnt > 2u * k) { gw[eb + 2 * k] = r.c[k].x; gy[eb + 2 * k] = r.c[k].y; gq[eb + 2 * k] = q; }
+        if (r.cnt > 2u * k + 1) { gw[eb + 2 * k + 1] = r.c[k].z; gy[eb + 2 * k + 1] = r.c[k].w; gq[eb + 2 * k + 1] = q; }
+    }
+    for (uint32_t j = REC_PAIRS; j < r.cnt; j++) {
+        gw[eb + j] = adj[r.adj + j];
+        gy[eb + j] = adjY[r.adj + j];
+        gq[eb + j] = q;
+    }
+}
+
 // Appends node v's fanout edges (fanout, other fanin, source entry q) at
 // position eb of an LDS edge list: chunk 0 comes with the header, chunks 1..5
 // are read as the count needs them, fanouts past REC_PAIRS from adj / adjY.
@@ -348,17 +416,17 @@ __device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *
 }
 
 // Consecutive narrow waves inside one workgroup, with every hand-over between
-// waves kept in LDS so that a wave costs about one dependent trip to memory:
+// waves kept in LDS so that a wave costs about two dependent trips to memory:
 //   - the frontier entries (node, fanin literals, pre-inserted slot and where
 //     each fanin's representative comes from) and the frontier's fanout edges
 //     (fanout, its other fanin, source entry) are in LDS, written by the
 //     thread that released the node;
 //   - phase A: one thread per node hashes it (fanin representatives from LDS,
-//     one returning atomic on the table), and one thread per fanout edge
-//     decrements the fanout's pending count while loading its NodeRec and the
-//     representative of its other fanin, all at once; a fanout that becomes
-//     ready enters the next frontier (its other fanin is resolved, or is in
-//     this wave, whose frontier position its rl then holds);
+//     one returning atomic on the table) while, on other threads, one thread
+//     per fanout edge decrements the fanout's pending count; a fanout that
+//     becomes ready reads its whole NodeRec and the representative of its
+//     other fanin in one trip and enters the next frontier (its other fanin is
+//     resolved, or is in this wave, whose frontier position its rl then holds);
 //   - phase B: each node's representative is the table minimum: a value from
 //     an earlier wave, returned by its own atomic, is final; otherwise it is the
 //     smallest id among this wave's nodes on the same slot, kept in an LDS table
@@ -460,32 +528,35 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
                 lev = 1u + (la > lb ? la : lb);
                 const unsigned long long val = (static_cast<unsigned long long>(k) << 32) | v;
                 const uint32_t r0 = raw0 < raw1 ? raw0 : raw1, r1 = raw0 < raw1 ? raw1 : raw0;
-                uint32_t slot = e_slot[b][tid];
-                if (!(l0 == r0 && l1 == r1 && slot != NO_SLOT)) {
+                uint32_t loc = e_slot[b][tid];
+                if (!(l0 == r0 && l1 == r1 && loc != NO_SLOT)) {
                     // fanins changed: probe for the mapped key
                     const uint64_t key = and_key(l0, l1);
                     uint64_t s = mix64(key) & a.mask;
-                    slot = NO_SLOT;
+                    loc = NO_SLOT;
                     for (uint64_t probe = 0; probe <= a.mask; ++probe) {
                         unsigned long long prev = KEY_EMPTY;
                         __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key,
                                                              __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
+                        if (prev == KEY_EMPTY || prev == key) {
+                            loc = loc_of_slot<__HIP_MEMORY_SCOPE_WORKGROUP>(a, s);
+                            break;
+                        }
                         s = (s + 1) & a.mask;
                     }
-                    if (slot == NO_SLOT) { atomicOr(a.err, 1u); slot = 0; }
-                    a.slotOf[v] = slot;
+                    if (loc == NO_SLOT) { atomicOr(a.err, 1u); loc = LOC_HT; }
+                    a.locOf[v] = loc;
                 }
-                const unsigned long long prev = __hip_atomic_fetch_min(&a.ht[2 * (size_t)slot + 1], val,
-                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const unsigned long long prev = __hip_atomic_fetch_min(vptr(a, loc), val, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (prev != KEY_EMPTY && (uint32_t)(prev >> 32) < k) {
                     fin = true;                    // an earlier wave holds the slot
                     r = 2u * (uint32_t)prev;
                 } else {
-                    // smallest id of this wave on the slot
-                    const unsigned long long dkey = (static_cast<unsigned long long>(k) << 32) | slot;
-                    uint32_t h = (slot * 0x9E3779B1u) >> (32 - WL_DT_BITS);
+                    // smallest id of this wave on the value location
+                    const unsigned long long dkey = (static_cast<unsigned long long>(k) << 32) | loc;
+                    uint32_t h = (loc * 0x9E3779B1u) >> (32 - WL_DT_BITS);
                     for (int probe = 0; probe < WL_DT; probe++) {
                         unsigned long long cur = __hip_atomic_load(&d_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if ((uint32_t)(cur >> 32) != k) {
@@ -502,8 +573,10 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
                 }
             }
         }
-        // ---- phase A: release the fanout edges
-        for (uint32_t t = tid; t < ne; t += WG1) {
+        // ---- phase A: release the fanout edges.  Edge t runs on thread
+        // (t + cn) mod WG1, so the threads hashing a node take edges only
+        // when nodes and edges together outnumber the threads.
+        for (uint32_t t = (tid + WG1 - cn) % WG1; t < ne; t += WG1) {
             const uint32_t w = g_w[b][t];
             const uint32_t left = __hip_atomic_fetch_add(&a.pending[w], 0xFFFFFFFFu, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -511,7 +584,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             // w is ready: its record header and its other fanin's representative
             const uint32_t y = g_y[b][t], q = g_q[b][t];
             const uint32_t x = e_v[b][q];
-            const RecHead rw = load_head(recs, w);
+            const RecFull rw = load_full(recs, w);
             const bool yLoad = y != x && y > (uint32_t)nPIs;
             const unsigned long long ry = yLoad ? a.rl[y] : 0ull;
             const uint32_t p = atomicAdd(&s_cnt[nb], 1u);
@@ -527,7 +600,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             e_s1[nb][p] = LID(rw.f1) == x ? sx : sy;
             a.rl[w] = (unsigned long long)INVALID32 | (unsigned long long)p << 32;
             const uint32_t eb = atomicAdd(&s_ecnt[nb], rw.cnt);
-            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges(g_w[nb], g_y[nb], g_q[nb], eb, rw, recs, w, (uint16_t)p, a.adj, adjY);
+            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges_full(g_w[nb], g_y[nb], g_q[nb], eb, rw, (uint16_t)p, a.adj, adjY);
             else s_over = 1;
         }
         __syncthreads();
@@ -577,8 +650,8 @@ __global__ void k_wave_of_order(uint32_t processed, uint32_t depth, const uint32
 }
 
 __global__ void k_resolve_all(int32_t n, int32_t nPIs, const uint8_t *__restrict__ dead, unsigned long long *rl,
-                              const uint32_t *__restrict__ slotOf, const unsigned long long *__restrict__ ht,
-                              uint8_t *surv) {
+                              const uint32_t *__restrict__ locOf, const unsigned long long *__restrict__ ht,
+                              const unsigned long long *__restrict__ val, uint8_t *surv) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = tid; v < n; v += stride) {
         uint8_t s = 0;
@@ -586,7 +659,9 @@ __global__ void k_resolve_all(int32_t n, int32_t nPIs, const uint8_t *__restrict
             unsigned long long x = rl[v];
             uint32_t r = (uint32_t)x;
             if (r == INVALID32) {
-                r = 2u * (uint32_t)(ht[2 * (size_t)slotOf[v] + 1] & 0xFFFFFFFFull);
+                const uint32_t loc = locOf[v];
+                const unsigned long long w = (loc & LOC_HT) ? ht[2 * (size_t)(loc & ~LOC_HT) + 1] : val[loc];
+                r = 2u * (uint32_t)(w & 0xFFFFFFFFull);
                 rl[v] = (x & 0xFFFFFFFF00000000ull) | r;
             }
             s = (r == 2u * (uint32_t)v);
@@ -706,16 +781,19 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     uint32_t *survIds = t.get<uint32_t>(n), *survWave = t.get<uint32_t>(n);
     uint32_t *sortIds = t.get<uint32_t>(n), *sortWave = t.get<uint32_t>(n), *d_count = t.get<uint32_t>(4);
     uint8_t *dead = t.get<uint8_t>(n), *surv = t.get<uint8_t>(n);
-    uint32_t *slotOf = t.get<uint32_t>(n), *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
+    uint32_t *locOf = t.get<uint32_t>(n), *slotTmp = t.get<uint32_t>(n);
+    uint32_t *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
+    unsigned long long *val = t.get<unsigned long long>(n);
     uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
     NodeRec *recs = t.get<NodeRec>(n);
     SweepCtl *ctl = t.get<SweepCtl>(1);
     // room for every live node's own key plus a changed key per node
     uint64_t cap = 1024;
     while (cap < 4ull * (uint64_t)n) cap <<= 1;
+    if (cap > (1ull << 31)) return gls_fail(ctx, GLS_ERR_ARG, "strash: %d objects exceed the table's slot range", n);
     unsigned long long *ht = t.get<unsigned long long>(2 * cap);
     if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
-        !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !slotOf ||
+        !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !locOf || !slotTmp || !val ||
         !fcnt || !fstart || !d_misc || !ctl || !ht || !recs || !adjY)
         return gls_fail(ctx, GLS_ERR_NOMEM, "strash: device allocation failed (n=%d)", n);
 
@@ -725,6 +803,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipMemsetAsync(rl, 0xFF, sizeof(unsigned long long) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(wave, 0, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(ht, 0xFF, sizeof(unsigned long long) * 2 * cap, st));
+    GLS_HIP(ctx, hipMemsetAsync(val, 0xFF, sizeof(unsigned long long) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(SweepCtl), st));
     const int gN = grid_for(n);
     SweepCtl h_ctl{};
@@ -756,13 +835,14 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
     k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj, adjY);
     k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
-    k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotOf, &d_misc[1]);
-    k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, slotOf, adjOff, adj, adjY, recs);
+    k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotTmp, &d_misc[1]);
+    k_canon<<<gN, kBlock, 0, st>>>(n, nPIs, dead, ht, slotTmp, locOf);
+    k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, locOf, adjOff, adj, adjY, recs);
     GLS_HIP(ctx, hipGetLastError());
 
     // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
     //    ones as one grid-wide kernel each
-    WaveArgs a{in.f0, in.f1, nPIs, rl, slotOf, wave, ht, cap - 1, adjOff, adj, pending, order,
+    WaveArgs a{in.f0, in.f1, nPIs, rl, locOf, wave, ht, val, cap - 1, adjOff, adj, pending, order,
                fcnt, fstart, &d_misc[0], &d_misc[1]};
     const uint32_t kThresh = WL_FM;
     uint32_t k = 1, waves = 0, cur = 0;
@@ -805,7 +885,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
         k_wave_of_order<<<grid_for(processed), kBlock, 0, st>>>(processed, (uint32_t)depth, fstart, order, wave);
     if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(n, nPIs, wave, tagIn, tagDepth);
     // 4. survivors in (wave, id) order
-    k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, slotOf, ht, surv);
+    k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, locOf, ht, val, surv);
     GLS_HIP(ctx, hipGetLastError());
     size_t tb2 = 0;
     rocprim::counting_iterator<uint32_t> ids(0u);
