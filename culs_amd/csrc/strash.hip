@@ -310,17 +310,17 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
     }
 }
 
-// Packed per-node record for the in-workgroup sweep: fanin literals, the
-// pre-inserted table slot, the fanout count, where the fanout list starts in
-// adj, then the first REC_PAIRS fanouts, each followed by its other fanin.
-// One 128-byte line per node: a node entering the frontier reads the 32-byte
-// header and only the 16-byte pair chunks its fanout count needs.
-constexpr int REC_PAIRS = 12;
-struct alignas(128) NodeRec {
-    uint32_t f0, f1, slot, cnt, adj, pad0, pad1, pad2;
+// Packed per-node record for the in-workgroup sweep, one 64-byte line:
+// fanin literals, value location, fanout count, then fanouts with their other
+// fanin (the fanin that is not this node).  Up to REC_PAIRS fanouts are
+// inline; with more, the first REC_PAIRS - 1 are inline and the last pair's
+// first word holds where the fanout list starts in adj / adjY.
+constexpr int REC_PAIRS = 6;
+struct alignas(64) NodeRec {
+    uint32_t f0, f1, loc, cnt;
     uint32_t wy[2 * REC_PAIRS];     // fanout j: wy[2j], its other fanin: wy[2j + 1]
 };
-static_assert(sizeof(NodeRec) == 128 && REC_PAIRS == 12, "one cache line per record, 6 pair chunks");
+static_assert(sizeof(NodeRec) == 64 && REC_PAIRS == 6, "one 64-byte record, read as four 16-byte loads");
 
 __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                             const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
@@ -332,85 +332,50 @@ __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict_
         NodeRec r;
         r.f0 = f0[v];
         r.f1 = f1[v];
-        r.slot = locOf[v];
-        r.adj = adjOff[v];
-        r.cnt = adjOff[v + 1] - r.adj;
-        r.pad0 = r.pad1 = r.pad2 = 0;
+        r.loc = locOf[v];
+        const uint32_t e0 = adjOff[v];
+        r.cnt = adjOff[v + 1] - e0;
+        const uint32_t inl = r.cnt <= (uint32_t)REC_PAIRS ? r.cnt : REC_PAIRS - 1;
 #pragma unroll
         for (int j = 0; j < REC_PAIRS; j++) {
-            r.wy[2 * j] = (uint32_t)j < r.cnt ? adj[r.adj + j] : 0u;
-            r.wy[2 * j + 1] = (uint32_t)j < r.cnt ? adjY[r.adj + j] : 0u;
+            r.wy[2 * j] = (uint32_t)j < inl ? adj[e0 + j] : 0u;
+            r.wy[2 * j + 1] = (uint32_t)j < inl ? adjY[e0 + j] : 0u;
         }
+        if (r.cnt > (uint32_t)REC_PAIRS) r.wy[2 * REC_PAIRS - 2] = e0;
         recs[v] = r;
     }
 }
 
-// Header of a record and its first pair chunk (fanouts 0 and 1).
-struct RecHead {
-    uint32_t f0, f1, slot, cnt, adj;
-    uint4 c0;
-};
-
-__device__ __forceinline__ RecHead load_head(const NodeRec *recs, uint32_t v) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
-    const uint4 h0 = p[0], h1 = p[1];
-    RecHead h;
-    h.f0 = h0.x; h.f1 = h0.y; h.slot = h0.z; h.cnt = h0.w; h.adj = h1.x;
-    h.c0 = p[2];
-    return h;
-}
-
-// A whole record: header and all six pair chunks, one trip.
 struct RecFull {
-    uint32_t f0, f1, slot, cnt, adj;
-    uint4 c[6];
+    uint32_t f0, f1, loc, cnt;
+    uint4 c[3];                     // pairs 2k, 2k + 1 in c[k]
 };
 
 __device__ __forceinline__ RecFull load_full(const NodeRec *recs, uint32_t v) {
     const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
-    const uint4 h0 = p[0], h1 = p[1];
+    const uint4 h = p[0];
     RecFull r;
-    r.f0 = h0.x; r.f1 = h0.y; r.slot = h0.z; r.cnt = h0.w; r.adj = h1.x;
+    r.f0 = h.x; r.f1 = h.y; r.loc = h.z; r.cnt = h.w;
 #pragma unroll
-    for (int k = 0; k < 6; k++) r.c[k] = p[2 + k];
+    for (int k = 0; k < 3; k++) r.c[k] = p[1 + k];
     return r;
 }
 
-// Appends the fanout edges of a node whose whole record is in registers.
-__device__ __forceinline__ void put_edges_full(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecFull &r,
-                                               uint16_t q, const uint32_t *__restrict__ adj,
-                                               const uint32_t *__restrict__ adjY) {
+// Appends a node's fanout edges (fanout, other fanin, source entry q) at
+// position eb of an LDS edge list.
+__device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecFull &r,
+                                          uint16_t q, const uint32_t *__restrict__ adj,
+                                          const uint32_t *__restrict__ adjY) {
+    const uint32_t inl = r.cnt <= (uint32_t)REC_PAIRS ? r.cnt : REC_PAIRS - 1;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        if (r.cnt > 2u * k) { gw[eb + 2 * k] = r.c[k].x; gy[eb + 2 * k] = r.c[k].y; gq[eb + 2 * k] = q; }
-        if (r.cnt > 2u * k + 1) { gw[eb + 2 * k + 1] = r.c[k].z; gy[eb + 2 * k + 1] = r.c[k].w; gq[eb + 2 * k + 1] = q; }
+    for (int k = 0; k < 3; k++) {
+        if (inl > 2u * k) { gw[eb + 2 * k] = r.c[k].x; gy[eb + 2 * k] = r.c[k].y; gq[eb + 2 * k] = q; }
+        if (inl > 2u * k + 1) { gw[eb + 2 * k + 1] = r.c[k].z; gy[eb + 2 * k + 1] = r.c[k].w; gq[eb + 2 * k + 1] = q; }
     }
-    for (uint32_t j = REC_PAIRS; j < r.cnt; j++) {
-        gw[eb + j] = adj[r.adj + j];
-        gy[eb + j] = adjY[r.adj + j];
-        gq[eb + j] = q;
-    }
-}
-
-// Appends node v's fanout edges (fanout, other fanin, source entry q) at
-// position eb of an LDS edge list: chunk 0 comes with the header, chunks 1..5
-// are read as the count needs them, fanouts past REC_PAIRS from adj / adjY.
-__device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecHead &h,
-                                          const NodeRec *recs, uint32_t v, uint16_t q,
-                                          const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(recs + v);
-    uint4 c[6];
-    c[0] = h.c0;
-#pragma unroll
-    for (int k = 1; k < 6; k++) c[k] = h.cnt > 2u * k ? p[2 + k] : make_uint4(0, 0, 0, 0);
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        if (h.cnt > 2u * k) { gw[eb + 2 * k] = c[k].x; gy[eb + 2 * k] = c[k].y; gq[eb + 2 * k] = q; }
-        if (h.cnt > 2u * k + 1) { gw[eb + 2 * k + 1] = c[k].z; gy[eb + 2 * k + 1] = c[k].w; gq[eb + 2 * k + 1] = q; }
-    }
-    for (uint32_t j = REC_PAIRS; j < h.cnt; j++) {
-        gw[eb + j] = adj[h.adj + j];
-        gy[eb + j] = adjY[h.adj + j];
+    const uint32_t e0 = r.c[2].z;   // list start when the count exceeds REC_PAIRS
+    for (uint32_t j = inl; j < r.cnt; j++) {
+        gw[eb + j] = adj[e0 + j];
+        gy[eb + j] = adjY[e0 + j];
         gq[eb + j] = q;
     }
 }
@@ -483,16 +448,16 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
     __syncthreads();
     for (uint32_t i = tid; i < cn; i += WG1) {
         const uint32_t v = e_v[0][i];
-        const RecHead r = load_head(recs, v);
+        const RecFull r = load_full(recs, v);
         const uint32_t u0 = LID(r.f0), u1 = LID(r.f1);
         e_f0[0][i] = r.f0;
         e_f1[0][i] = r.f1;
-        e_slot[0][i] = r.slot;
+        e_slot[0][i] = r.loc;
         // fanins are in earlier waves, all resolved
         e_s0[0][i] = u0 <= (uint32_t)nPIs ? 2ull * u0 : a.rl[u0];
         e_s1[0][i] = u1 <= (uint32_t)nPIs ? 2ull * u1 : a.rl[u1];
         const uint32_t eb = atomicAdd(&s_ecnt[0], r.cnt);
-        if (eb + r.cnt <= (uint32_t)WL_EM) put_edges(g_w[0], g_y[0], g_q[0], eb, r, recs, v, (uint16_t)i, a.adj, adjY);
+        if (eb + r.cnt <= (uint32_t)WL_EM) put_edges(g_w[0], g_y[0], g_q[0], eb, r, (uint16_t)i, a.adj, adjY);
         else s_over = 1;
     }
     __syncthreads();
@@ -595,12 +560,12 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             e_v[nb][p] = w;
             e_f0[nb][p] = rw.f0;
             e_f1[nb][p] = rw.f1;
-            e_slot[nb][p] = rw.slot;
+            e_slot[nb][p] = rw.loc;
             e_s0[nb][p] = LID(rw.f0) == x ? sx : sy;
             e_s1[nb][p] = LID(rw.f1) == x ? sx : sy;
             a.rl[w] = (unsigned long long)INVALID32 | (unsigned long long)p << 32;
             const uint32_t eb = atomicAdd(&s_ecnt[nb], rw.cnt);
-            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges_full(g_w[nb], g_y[nb], g_q[nb], eb, rw, (uint16_t)p, a.adj, adjY);
+            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges(g_w[nb], g_y[nb], g_q[nb], eb, rw, (uint16_t)p, a.adj, adjY);
             else s_over = 1;
         }
         __syncthreads();
