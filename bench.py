@@ -159,7 +159,8 @@ def main():
         avg_launch_s, bytes_per_launch, achieved = 0.0, n_in * B_CUT_PER_AND / nlev, 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                "kernel": "k_cuts_level", "launches_per_step": nlev, "timed_launches": ks,
+                "kernel": "k_cuts_xcd" if a.cut_mode == culs_amd.GLS_RW_XCD_CUTS else "k_cuts_level",
+                "launches_per_step": 1 if a.cut_mode == culs_amd.GLS_RW_XCD_CUTS else nlev, "timed_launches": ks,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
                 "bytes_per_node": B_CUT_PER_AND,
                 "pass_alg_GBs": round(n_in * B_RW_PER_AND / (total / a.steps) / 1e9, 3)}

@@ -481,6 +481,78 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t 
     }
 }
 
+// XCD-local schedule: one persistent kernel on the 32 CUs of one XCD.
+// Workgroups are dispatched round-robin over the XCDs, so workgroups 0, 8, 16,
+// ... share XCD 0 and its L2; the others return at once.  Between levels a
+// counter barrier (device-scope atomics, about 1 us) replaces a kernel
+// boundary (about 4.7 us on this path).  A CU's L1 is not coherent with the
+// other CUs' stores: after each barrier one wavefront of the workgroup runs
+// buffer_inv sc1, which drops the CU's L1 lines (buffer_inv sc0 does not in
+// this mode), so the fanin records written by other CUs are read from L2.
+// Every workgroup records its XCD; if they are not all on one, or a barrier
+// wait exceeds its bound, ctl[2] is set, every workgroup stops, and the host
+// recomputes the cut sets with one launch per level.
+constexpr int XCD_WG = 1024;
+constexpr int XCD_COUNT = 8;
+constexpr uint32_t XCD_SPIN_LIMIT = 1u << 20;
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xFu;
+}
+
+// ctl[0] barrier arrivals, ctl[1] mask of XCDs seen, ctl[2] error
+__device__ __forceinline__ bool xcd_barrier(uint32_t *ctl, uint32_t &epoch, uint32_t nwg, uint32_t *s_stop) {
+    __builtin_amdgcn_s_waitcnt(0);          // this wavefront's stores have reached L2
+    __syncthreads();
+    ++epoch;
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t target = epoch * nwg;
+        uint32_t spins = 0;
+        bool bad = false;
+        while (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > XCD_SPIN_LIMIT || __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                bad = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (bad) __hip_atomic_fetch_or(&ctl[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+        *s_stop = bad ? 1u : 0u;
+    }
+    __syncthreads();
+    return *s_stop == 0;
+}
+
+__global__ void __launch_bounds__(XCD_WG) k_cuts_xcd(int32_t l0, int32_t l1, const int32_t *__restrict__ starts,
+                                                     const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                                                     const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                     const uint8_t *__restrict__ sigBits, uint32_t nwg, uint32_t *ctl) {
+    if (blockIdx.x % XCD_COUNT != 0) return;
+    __shared__ uint32_t s_stop;
+    const uint32_t wg = blockIdx.x / XCD_COUNT;
+    if (wg >= nwg) return;
+    const int lane = threadIdx.x & 63;
+    const int gw = (int)wg * (XCD_WG / 64) + (int)(threadIdx.x >> 6);
+    const int nw = (int)nwg * (XCD_WG / 64);
+    uint32_t epoch = 0;
+    if (threadIdx.x == 0) __hip_atomic_fetch_or(&ctl[1], 1u << xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!xcd_barrier(ctl, epoch, nwg, &s_stop)) return;
+    if (__popc(__hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 1) {
+        if (threadIdx.x == 0) __hip_atomic_fetch_or(&ctl[2], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int l = l0; l < l1; l++) {
+        const int s = starts[l], e = starts[l + 1];
+        for (int v = s + gw; v < e; v += nw)
+            cut_node<REC_PLAIN>(v, lane, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
+        if (!xcd_barrier(ctl, epoch, nwg, &s_stop)) return;
+    }
+}
+
 // Dataflow schedule: persistent wavefronts take nodes in level-major id order
 // from a ticket counter and wait only for their two fanins' cut sets, polled
 // in the fence-free record format; no level waits for the whole previous
@@ -1221,6 +1293,47 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                             n - g.nPIs - 1);
         return GLS_OK;
     }
+    if (mode == GLS_RW_XCD_CUTS && L > 16) {
+        int dev = 0, nCU = 0;
+        GLS_HIP(ctx, hipGetDevice(&dev));
+        GLS_HIP(ctx, hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint32_t nwg = (uint32_t)(nCU / XCD_COUNT);
+        int32_t *d_starts = pool_get<int32_t>(ctx, levelStart.size());
+        uint32_t *ctl = pool_get<uint32_t>(ctx, 4);
+        if (!d_starts || !ctl) { pool_put(ctx, d_starts); pool_put(ctx, ctl); return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: xcd control"); }
+        GLS_HIP(ctx, hipMemcpyAsync(d_starts, levelStart.data(), sizeof(int32_t) * levelStart.size(),
+                                    hipMemcpyHostToDevice, st));
+        GLS_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(uint32_t) * 4, st));
+        const dim3 grid((unsigned)(nwg * XCD_COUNT));
+        if (kSampleLaunches) {
+            const size_t k = 2 * (size_t)ctx->ksamp_n;
+            while (ctx->kev.size() < k + 2) {
+                hipEvent_t ev = nullptr;
+                GLS_HIP(ctx, hipEventCreate(&ev));
+                ctx->kev.push_back(ev);
+            }
+            hipExtLaunchKernelGGL(k_cuts_xcd, grid, dim3(XCD_WG), 0, st, ctx->kev[k], ctx->kev[k + 1], 0, 1, L,
+                                  (const int32_t *)d_starts, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
+                                  (const int32_t *)d_nref, d_cuts, sigBits, nwg, ctl);
+            ctx->ksamp_units.push_back(levelStart[L] - levelStart[1]);
+            ctx->ksamp_n++;
+        } else {
+            k_cuts_xcd<<<grid, XCD_WG, 0, st>>>(1, L, d_starts, g.f0, g.f1, d_nref, d_cuts, sigBits, nwg, ctl);
+        }
+        GLS_HIP(ctx, hipGetLastError());
+        uint32_t hctl[3] = {0, 0, 0};
+        const int rc = stream_copy(ctx, hctl, ctl, sizeof(hctl), hipMemcpyDeviceToHost);
+        pool_put(ctx, d_starts);
+        pool_put(ctx, ctl);
+        if (rc != GLS_OK) return rc;
+        if (hctl[2] == 0) return GLS_OK;
+        // not all on one XCD, or a wait overran: redo with one launch per level
+        ctx->ksamp_n = 0;
+        ctx->ksamp_units.clear();
+        GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
+        if (g.nPIs > 0) k_cuts_pi<<<blocks(g.nPIs, kBlock), kBlock, 0, st>>>(g.nPIs, d_nref, d_cuts);
+        mode = 0;
+    }
     bool coop = mode == GLS_RW_COOP_CUTS && L > 16;
     int G = 0;
     if (coop) {
@@ -1351,7 +1464,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             if (!sigBits) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: signature bits"); }
             GLS_TRY(sig_bits_for_tags(ctx, n, tagR, sigBits));
         }
-        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref, flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS),
+        GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
+                                   flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS),
                                    sigBits));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};

@@ -107,9 +107,12 @@ int gls_strash(gls_aig *aig, int32_t flags);
  * the device; the result is strashed.  flags select how cut enumeration is
  * scheduled (the results are identical): 0 = one kernel launch per level,
  * GLS_RW_COOP_CUTS = one cooperative launch with a grid barrier per level,
- * GLS_RW_DATAFLOW_CUTS = persistent wavefronts that wait only for their fanins. */
+ * GLS_RW_DATAFLOW_CUTS = persistent wavefronts that wait only for their fanins,
+ * GLS_RW_XCD_CUTS = one persistent kernel on the CUs of one XCD with a barrier
+ * per level (falls back to one launch per level where that cannot run). */
 #define GLS_RW_COOP_CUTS 2
 #define GLS_RW_DATAFLOW_CUTS 4
+#define GLS_RW_XCD_CUTS 8
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing
