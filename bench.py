@@ -145,8 +145,9 @@ def main():
     depth_out = out.stats().levels
 
     # dominant kernel: the per-level cut enumeration kernel (k_cuts_level).  Every
-    # 16th launch carries HIP events on its own dispatch (the context stream), so
-    # the average is the kernel's execution time as rocprofv3 reports it.
+    # 64 levels a window of 8 consecutive launches is timed with HIP events on
+    # the first and last dispatch (the context stream), so the average is the
+    # per-launch kernel time as rocprofv3 reports it.
     ks = sum(i["kernel_samples"] for i in infos)
     kms = sum(i["kernel_sample_ms"] for i in infos)
     knodes = sum(i["kernel_sample_nodes"] for i in infos)

@@ -56,9 +56,10 @@ struct gls_ctx {
     uint8_t *d_lib_roots = nullptr;
     hipEvent_t ev[16] = {};
     bool lib_ready = false;
-    // sampled per-launch timing: event pairs attached to kernel dispatches
+    // sampled launch timing: one event pair per window of consecutive dispatches
     std::vector<hipEvent_t> kev;      // 2 per sample, created on demand
-    std::vector<int64_t> ksamp_units; // units (nodes) of each sampled launch
+    std::vector<int64_t> ksamp_units; // units (nodes) of each sampled window
+    std::vector<int64_t> ksamp_launches; // launches in each sampled window
     int ksamp_n = 0;
 };
 

@@ -1240,10 +1240,11 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
     return GLS_OK;
 }
 
-// every KSAMPLE_EVERY-th per-level cut launch is timed on its own dispatch
+// per-level cut launches are timed in windows (see dev_enumerate_cuts)
 // (GLS_KERNEL_SAMPLE=0 in the environment turns this off, e.g. under a
 // profiler that collects counters per dispatch)
-static constexpr int KSAMPLE_EVERY = 16;
+static constexpr int KSAMPLE_EVERY = 64;
+static constexpr int KSAMPLE_WIN = 8;
 static const bool kSampleLaunches = [] {
     const char *e = getenv("GLS_KERNEL_SAMPLE");
     return !(e && e[0] == '0');
@@ -1255,6 +1256,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     const int32_t n = g.nObjs;
     ctx->ksamp_n = 0;
     ctx->ksamp_units.clear();
+    ctx->ksamp_launches.clear();
     GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
     k_nref<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f0, g.f1, g.nPOs, g.pos, d_nref);
@@ -1316,6 +1318,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                                   (const int32_t *)d_starts, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
                                   (const int32_t *)d_nref, d_cuts, sigBits, nwg, ctl);
             ctx->ksamp_units.push_back(levelStart[L] - levelStart[1]);
+            ctx->ksamp_launches.push_back(1);
             ctx->ksamp_n++;
         } else {
             k_cuts_xcd<<<grid, XCD_WG, 0, st>>>(1, L, d_starts, g.f0, g.f1, d_nref, d_cuts, sigBits, nwg, ctl);
@@ -1330,6 +1333,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // not all on one XCD, or a wait overran: redo with one launch per level
         ctx->ksamp_n = 0;
         ctx->ksamp_units.clear();
+        ctx->ksamp_launches.clear();
         GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
         if (g.nPIs > 0) k_cuts_pi<<<blocks(g.nPIs, kBlock), kBlock, 0, st>>>(g.nPIs, d_nref, d_cuts);
         mode = 0;
@@ -1361,28 +1365,43 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         pool_put(ctx, d_starts);
         if (e != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cooperative cut launch: %s", hipGetErrorString(e));
     } else {
+        // sampled timing: a window of KSAMPLE_WIN consecutive launches starts
+        // every KSAMPLE_EVERY levels; its first dispatch carries the start event
+        // and its last the stop event, so the elapsed time over the launches is
+        // the per-launch kernel time a profiler reports, with the events' own
+        // cost spread over the window
+        int win = 0;
         for (int l = 1; l < L; l++) {
             int32_t s = levelStart[l], e = levelStart[l + 1];
             if (e <= s) continue;
             const dim3 grid(blocks(e - s, CUTW_BLOCK / 64));
-            if (kSampleLaunches && l % KSAMPLE_EVERY == 0) {
-                // this launch carries events on its own dispatch: their elapsed
-                // time is the kernel's execution time, as a profiler reports it
+            hipEvent_t evStart = nullptr, evStop = nullptr;
+            if (kSampleLaunches && win == 0 && l % KSAMPLE_EVERY == 0) {
                 const size_t k = 2 * (size_t)ctx->ksamp_n;
                 while (ctx->kev.size() < k + 2) {
                     hipEvent_t ev = nullptr;
                     GLS_HIP(ctx, hipEventCreate(&ev));
                     ctx->kev.push_back(ev);
                 }
-                hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTW_BLOCK), 0, st, ctx->kev[k], ctx->kev[k + 1], 0,
-                                      s, e - s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
-                                      (const int32_t *)d_nref, d_cuts, sigBits);
-                ctx->ksamp_units.push_back(e - s);
+                evStart = ctx->kev[k];
+                ctx->ksamp_units.push_back(0);
+                ctx->ksamp_launches.push_back(0);
                 ctx->ksamp_n++;
+                win = KSAMPLE_WIN;
+            }
+            if (win > 0) {
+                ctx->ksamp_units.back() += e - s;
+                ctx->ksamp_launches.back() += 1;
+                if (--win == 0) evStop = ctx->kev[2 * (size_t)ctx->ksamp_n - 1];
+                hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTW_BLOCK), 0, st, evStart, evStop, 0, s, e - s,
+                                      (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
+                                      d_cuts, sigBits);
             } else {
                 k_cuts_level<<<grid, CUTW_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
         }
+        // a window cut short by the last level ends after its last launch
+        if (win > 0) GLS_HIP(ctx, hipEventRecord(ctx->kev[2 * (size_t)ctx->ksamp_n - 1], st));
     }
     GLS_HIP(ctx, hipGetLastError());
     return GLS_OK;
@@ -1585,14 +1604,16 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->levels_enumerated = (int64_t)levelStart.size() - 2;
             info->est_gain = (int64_t)estGain;
             double kms = 0;
-            int64_t kunits = 0;
+            int64_t kunits = 0, klaunches = 0;
             for (int i = 0; i < ctx->ksamp_n; i++) {
                 float x = 0;
-                if (hipEventElapsedTime(&x, ctx->kev[2 * i], ctx->kev[2 * i + 1]) == hipSuccess) kms += x;
+                if (hipEventElapsedTime(&x, ctx->kev[2 * i], ctx->kev[2 * i + 1]) != hipSuccess) continue;
+                kms += x;
                 kunits += ctx->ksamp_units[i];
+                klaunches += ctx->ksamp_launches[i];
             }
             info->kernel_sample_ms = kms;
-            info->kernel_samples = ctx->ksamp_n;
+            info->kernel_samples = klaunches;
             info->kernel_sample_nodes = kunits;
         }
     }

@@ -52,9 +52,10 @@ typedef struct {
     double reorder_ms, cuts_ms, table_ms, eval_ms, select_ms, build_ms, strash_ms;
     int64_t candidates, selected, new_nodes, rounds, levels_enumerated;
     int64_t est_gain;
-    /* Sampled launch timing of the per-level cut kernel: every 16th launch
-     * carries HIP start/stop events on its own dispatch; the sum of those
-     * launches' kernel durations, their count and the nodes they processed. */
+    /* Sampled launch timing of the cut kernel: every 64 levels a window of 8
+     * consecutive launches, the first dispatch carrying a HIP start event and
+     * the last a stop event; the windows' summed elapsed time, the launches
+     * they cover and the nodes those launches processed. */
     double kernel_sample_ms;
     int64_t kernel_samples, kernel_sample_nodes;
 } gls_op_info;
