@@ -64,38 +64,49 @@ __global__ void k_dangle_round(uint32_t k, SweepCtl *ctl, int32_t nPIs, const ui
     sweep_round_end(ctl, k);
 }
 
+constexpr uint32_t PEND_ONE = 0x101u;
+
 __global__ void k_kahn_count(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                              const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
                              uint32_t *cnt, uint32_t *pending) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
-        if (dead[v]) continue;
+        if (dead[v]) { pending[v] = 0; continue; }
         uint32_t a = LID(f0[v]), b = LID(f1[v]);
         uint32_t p = 0;
         if (a > (uint32_t)nPIs) { atomicAdd(&cnt[a], 1u); ++p; }
         if (b > (uint32_t)nPIs) { atomicAdd(&cnt[b], 1u); ++p; }
-        pending[v] = p;
+        // a node with one non-PI fanin carries PEND_ONE: the in-workgroup sweep
+        // releases it without touching the counter, and the counter's low byte
+        // is the count
+        pending[v] = p == 1 ? PEND_ONE : p;
     }
 }
 
 // Fanout CSR: adj holds the fanout, adjY its other fanin (the fanin that is
-// not the list's owner).
+// not the list's owner), with Y_OFL set when the fanout has more fanouts of
+// its own than its node record holds (so its overflow record is read with it).
+constexpr uint32_t Y_OFL = 0x80000000u;
+constexpr uint32_t Y_ID = 0x7FFFFFFFu;
+constexpr uint32_t REC_INLINE = 6;      // fanouts held in a node record
+
 __global__ void k_kahn_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                             const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
-                            uint32_t *cursor, uint32_t *adj, uint32_t *adjY) {
+                            const uint32_t *__restrict__ cnt, uint32_t *cursor, uint32_t *adj, uint32_t *adjY) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
         if (dead[v]) continue;
         uint32_t a = LID(f0[v]), b = LID(f1[v]);
+        const uint32_t ofl = cnt[v] > REC_INLINE ? Y_OFL : 0u;
         if (a > (uint32_t)nPIs) {
             const uint32_t e = atomicAdd(&cursor[a], 1u);
             adj[e] = (uint32_t)v;
-            adjY[e] = b;
+            adjY[e] = b | ofl;
         }
         if (b > (uint32_t)nPIs) {
             const uint32_t e = atomicAdd(&cursor[b], 1u);
             adj[e] = (uint32_t)v;
-            adjY[e] = a;
+            adjY[e] = a | ofl;
         }
     }
 }
@@ -273,7 +284,7 @@ __device__ __forceinline__ void release_range(const WaveArgs &a, uint32_t e0, ui
             r[j] = (e + j < ee) ? __hip_atomic_fetch_add(&a.pending[w[j]], 0xFFFFFFFFu, __ATOMIC_RELAXED, SCOPE) : 0u;
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if (e + j < ee && r[j] == 1u) emit(w[j]);
+            if (e + j < ee && (r[j] & 0xFFu) == 1u) emit(w[j]);
     }
 }
 
@@ -311,21 +322,27 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
 }
 
 // Packed per-node record for the in-workgroup sweep, one 64-byte line:
-// fanin literals, value location, fanout count, then fanouts with their other
-// fanin (the fanin that is not this node).  Up to REC_PAIRS fanouts are
-// inline; with more, the first REC_PAIRS - 1 are inline and the last pair's
-// first word holds where the fanout list starts in adj / adjY.
+// fanin literals, value location, fanout count, then the first REC_INLINE
+// fanouts with their other fanin (adjY's word, flag included).  A node with
+// more fanouts also has an overflow record: fanouts 6..11 and where its
+// fanout list starts in adj / adjY (for the rest).
 constexpr int REC_PAIRS = 6;
 struct alignas(64) NodeRec {
     uint32_t f0, f1, loc, cnt;
     uint32_t wy[2 * REC_PAIRS];     // fanout j: wy[2j], its other fanin: wy[2j + 1]
 };
-static_assert(sizeof(NodeRec) == 64 && REC_PAIRS == 6, "one 64-byte record, read as four 16-byte loads");
+struct alignas(64) NodeOfl {
+    uint32_t wy[2 * REC_PAIRS];     // fanouts 6..11
+    uint32_t start, pad0, pad1, pad2;
+};
+static_assert(sizeof(NodeRec) == 64 && sizeof(NodeOfl) == 64 && REC_PAIRS == (int)REC_INLINE,
+              "64-byte records, read as four 16-byte loads");
 
 __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
                             const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
                             const uint32_t *__restrict__ locOf, const uint32_t *__restrict__ adjOff,
-                            const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY, NodeRec *recs) {
+                            const uint32_t *__restrict__ adj, const uint32_t *__restrict__ adjY, NodeRec *recs,
+                            NodeOfl *ofls) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
         if (dead[v]) continue;
@@ -335,20 +352,34 @@ __global__ void k_node_recs(int32_t n, int32_t nPIs, const uint32_t *__restrict_
         r.loc = locOf[v];
         const uint32_t e0 = adjOff[v];
         r.cnt = adjOff[v + 1] - e0;
-        const uint32_t inl = r.cnt <= (uint32_t)REC_PAIRS ? r.cnt : REC_PAIRS - 1;
 #pragma unroll
         for (int j = 0; j < REC_PAIRS; j++) {
-            r.wy[2 * j] = (uint32_t)j < inl ? adj[e0 + j] : 0u;
-            r.wy[2 * j + 1] = (uint32_t)j < inl ? adjY[e0 + j] : 0u;
+            r.wy[2 * j] = (uint32_t)j < r.cnt ? adj[e0 + j] : 0u;
+            r.wy[2 * j + 1] = (uint32_t)j < r.cnt ? adjY[e0 + j] : 0u;
         }
-        if (r.cnt > (uint32_t)REC_PAIRS) r.wy[2 * REC_PAIRS - 2] = e0;
         recs[v] = r;
+        if (r.cnt > REC_INLINE) {
+            NodeOfl o;
+#pragma unroll
+            for (int j = 0; j < REC_PAIRS; j++) {
+                const uint32_t jj = REC_INLINE + (uint32_t)j;
+                o.wy[2 * j] = jj < r.cnt ? adj[e0 + jj] : 0u;
+                o.wy[2 * j + 1] = jj < r.cnt ? adjY[e0 + jj] : 0u;
+            }
+            o.start = e0;
+            o.pad0 = o.pad1 = o.pad2 = 0;
+            ofls[v] = o;
+        }
     }
 }
 
 struct RecFull {
     uint32_t f0, f1, loc, cnt;
     uint4 c[3];                     // pairs 2k, 2k + 1 in c[k]
+};
+struct OflFull {
+    uint4 c[3];                     // pairs 6 + 2k, 7 + 2k in c[k]
+    uint32_t start;
 };
 
 __device__ __forceinline__ RecFull load_full(const NodeRec *recs, uint32_t v) {
@@ -361,21 +392,36 @@ __device__ __forceinline__ RecFull load_full(const NodeRec *recs, uint32_t v) {
     return r;
 }
 
+__device__ __forceinline__ OflFull load_ofl(const NodeOfl *ofls, uint32_t v) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(ofls + v);
+    OflFull o;
+#pragma unroll
+    for (int k = 0; k < 3; k++) o.c[k] = p[k];
+    o.start = p[3].x;
+    return o;
+}
+
 // Appends a node's fanout edges (fanout, other fanin, source entry q) at
-// position eb of an LDS edge list.
+// position eb of an LDS edge list: fanouts 0..5 from its record, 6..11 from
+// its overflow record o (read when the count exceeds 6), the rest from adj.
 __device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *gq, uint32_t eb, const RecFull &r,
-                                          uint16_t q, const uint32_t *__restrict__ adj,
+                                          const OflFull &o, uint16_t q, const uint32_t *__restrict__ adj,
                                           const uint32_t *__restrict__ adjY) {
-    const uint32_t inl = r.cnt <= (uint32_t)REC_PAIRS ? r.cnt : REC_PAIRS - 1;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        if (inl > 2u * k) { gw[eb + 2 * k] = r.c[k].x; gy[eb + 2 * k] = r.c[k].y; gq[eb + 2 * k] = q; }
-        if (inl > 2u * k + 1) { gw[eb + 2 * k + 1] = r.c[k].z; gy[eb + 2 * k + 1] = r.c[k].w; gq[eb + 2 * k + 1] = q; }
+        if (r.cnt > 2u * k) { gw[eb + 2 * k] = r.c[k].x; gy[eb + 2 * k] = r.c[k].y; gq[eb + 2 * k] = q; }
+        if (r.cnt > 2u * k + 1) { gw[eb + 2 * k + 1] = r.c[k].z; gy[eb + 2 * k + 1] = r.c[k].w; gq[eb + 2 * k + 1] = q; }
     }
-    const uint32_t e0 = r.c[2].z;   // list start when the count exceeds REC_PAIRS
-    for (uint32_t j = inl; j < r.cnt; j++) {
-        gw[eb + j] = adj[e0 + j];
-        gy[eb + j] = adjY[e0 + j];
+    if (r.cnt <= REC_INLINE) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t j = REC_INLINE + 2u * k;
+        if (r.cnt > j) { gw[eb + j] = o.c[k].x; gy[eb + j] = o.c[k].y; gq[eb + j] = q; }
+        if (r.cnt > j + 1) { gw[eb + j + 1] = o.c[k].z; gy[eb + j + 1] = o.c[k].w; gq[eb + j + 1] = q; }
+    }
+    for (uint32_t j = 2 * REC_INLINE; j < r.cnt; j++) {
+        gw[eb + j] = adj[o.start + j];
+        gy[eb + j] = adjY[o.start + j];
         gq[eb + j] = q;
     }
 }
@@ -408,6 +454,19 @@ constexpr int WL_DT = 1 << WL_DT_BITS;
 constexpr unsigned long long SRC_POS = 1ull << 63;
 static_assert(WL_FM <= WG1 && WL_EM <= 2 * WG1, "one node and two edges per thread");
 static_assert(WL_FM <= 65536, "entry index in 16 bits");
+// Fanout counters of a window of ids kept in LDS as one bit each: a node with
+// two non-PI fanins is ready on its second release, and its bit toggles on
+// each (PEND_ONE nodes are ready on their only release).  The window is
+// circular (bit of id w at w mod PB_WIN) and moves in steps of PB_STEP ids;
+// ids outside it use the global counters.  Ownership of an arrival moves with
+// the window: an id leaving it with its bit set gets one global decrement; an
+// id entering it with one global arrival (counter 1) gets its bit set and
+// its counter back to 2.  At exit every set bit is written back.
+constexpr int PB_WORDS = 3840;
+constexpr uint32_t PB_WIN = (uint32_t)PB_WORDS * 32u;   // 122,880 ids
+constexpr uint32_t PB_STEP = PB_WIN / 4;
+constexpr uint32_t PB_MARGIN = 8192;    // ids kept below the frontier's smallest
+static_assert(PB_STEP % 32 == 0, "window steps by whole words");
 
 // Source of a fanin's representative: SRC_POS | frontier position of this
 // wave, or the resolved rl value (literal | level << 32).
@@ -415,9 +474,42 @@ __device__ __forceinline__ unsigned long long src_of(unsigned long long r) {
     return (uint32_t)r == INVALID32 ? (SRC_POS | (r >> 32)) : r;
 }
 
+// ids [lo, hi) leave the window: their set bits become one global decrement
+__device__ __forceinline__ void pb_release_range(uint32_t *s_pb, uint32_t lo, uint32_t hi, uint32_t *pending,
+                                                 int32_t n) {
+    for (uint32_t wd = lo / 32 + threadIdx.x; wd < hi / 32; wd += WG1) {
+        const uint32_t base = wd * 32;
+        uint32_t bits = s_pb[(base % PB_WIN) / 32];
+        s_pb[(base % PB_WIN) / 32] = 0;
+        while (bits) {
+            const uint32_t j = __ffs(bits) - 1;
+            bits &= bits - 1;
+            if (base + j < (uint32_t)n)
+                __hip_atomic_fetch_add(&pending[base + j], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ids [lo, hi) enter the window (their words are clear): an id with one
+// global arrival takes it over as its bit
+__device__ __forceinline__ void pb_take_range(uint32_t *s_pb, uint32_t lo, uint32_t hi, uint32_t *pending, int32_t n) {
+    for (uint32_t wd = lo / 32 + threadIdx.x; wd < hi / 32; wd += WG1) {
+        const uint32_t base = wd * 32;
+        uint32_t bits = 0;
+        for (uint32_t j = 0; j < 32; j++) {
+            const uint32_t w = base + j;
+            if (w < (uint32_t)n && pending[w] == 1u) {
+                bits |= 1u << j;
+                pending[w] = 2u;
+            }
+        }
+        s_pb[(base % PB_WIN) / 32] = bits;
+    }
+}
+
 __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh, WaveArgs a,
-                                                   const NodeRec *__restrict__ recs,
-                                                   const uint32_t *__restrict__ adjY, uint32_t *kOut) {
+                                                   const NodeRec *__restrict__ recs, const NodeOfl *__restrict__ ofls,
+                                                   const uint32_t *__restrict__ adjY, uint32_t *kOut, int32_t n) {
     __shared__ uint32_t e_v[2][WL_FM], e_f0[2][WL_FM], e_f1[2][WL_FM], e_slot[2][WL_FM];
     __shared__ unsigned long long e_s0[2][WL_FM], e_s1[2][WL_FM];
     __shared__ uint32_t g_w[2][WL_EM], g_y[2][WL_EM];
@@ -425,13 +517,18 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
     __shared__ unsigned long long s_rep[WL_FM];
     __shared__ unsigned long long d_key[WL_DT], d_val[WL_DT];
     __shared__ uint32_t s_cnt[2], s_ecnt[2], s_over;
+    __shared__ uint32_t s_pb[PB_WORDS];
+    __shared__ uint32_t s_pbase, s_fmin[2];
     const uint32_t tid = threadIdx.x;
     const int32_t nPIs = a.nPIs;
     uint32_t k = k0;
     const uint32_t pn = a.fcnt[k - 1], ps = a.fstart[k - 1];
     uint32_t cn = a.fcnt[k], cs = a.fstart[k];
     for (uint32_t i = tid; i < WL_DT; i += WG1) { d_key[i] = 0; d_val[i] = 0; }
-    if (tid == 0) { s_cnt[0] = 0; s_cnt[1] = 0; s_ecnt[0] = 0; s_ecnt[1] = 0; s_over = 0; }
+    if (tid == 0) {
+        s_cnt[0] = 0; s_cnt[1] = 0; s_ecnt[0] = 0; s_ecnt[1] = 0; s_over = 0;
+        s_fmin[0] = 0xFFFFFFFFu; s_fmin[1] = 0xFFFFFFFFu;
+    }
     // the previous wave may come from the grid-wide kernel, unresolved
     for (uint32_t i = tid; i < pn; i += WG1) resolve<__HIP_MEMORY_SCOPE_WORKGROUP>(a, a.order[ps + i]);
     __syncthreads();
@@ -444,11 +541,19 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         const uint32_t v = a.order[cs + i];
         e_v[0][i] = v;
         a.rl[v] = (unsigned long long)INVALID32 | (unsigned long long)i << 32;
+        atomicMin(&s_fmin[0], v);
     }
+    __syncthreads();
+    // counter window from the frontier's smallest id
+    const uint32_t pb0 = (s_fmin[0] > PB_MARGIN ? s_fmin[0] - PB_MARGIN : 0u) & ~31u;
+    pb_take_range(s_pb, pb0, pb0 + PB_WIN, a.pending, n);
+    if (tid == 0) { s_pbase = pb0; s_fmin[0] = 0xFFFFFFFFu; }
     __syncthreads();
     for (uint32_t i = tid; i < cn; i += WG1) {
         const uint32_t v = e_v[0][i];
         const RecFull r = load_full(recs, v);
+        OflFull o{};
+        if (r.cnt > REC_INLINE) o = load_ofl(ofls, v);
         const uint32_t u0 = LID(r.f0), u1 = LID(r.f1);
         e_f0[0][i] = r.f0;
         e_f1[0][i] = r.f1;
@@ -457,11 +562,12 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         e_s0[0][i] = u0 <= (uint32_t)nPIs ? 2ull * u0 : a.rl[u0];
         e_s1[0][i] = u1 <= (uint32_t)nPIs ? 2ull * u1 : a.rl[u1];
         const uint32_t eb = atomicAdd(&s_ecnt[0], r.cnt);
-        if (eb + r.cnt <= (uint32_t)WL_EM) put_edges(g_w[0], g_y[0], g_q[0], eb, r, (uint16_t)i, a.adj, adjY);
+        if (eb + r.cnt <= (uint32_t)WL_EM) put_edges(g_w[0], g_y[0], g_q[0], eb, r, o, (uint16_t)i, a.adj, adjY);
         else s_over = 1;
     }
     __syncthreads();
     if (s_over) {
+        pb_release_range(s_pb, s_pbase, s_pbase + PB_WIN, a.pending, n);
         if (tid == 0) *kOut = k;
         return;
     }
@@ -540,19 +646,73 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         }
         // ---- phase A: release the fanout edges.  Edge t runs on thread
         // (t + cn) mod WG1, so the threads hashing a node take edges only
-        // when nodes and edges together outnumber the threads.
-        for (uint32_t t = (tid + WG1 - cn) % WG1; t < ne; t += WG1) {
-            const uint32_t w = g_w[b][t];
-            const uint32_t left = __hip_atomic_fetch_add(&a.pending[w], 0xFFFFFFFFu, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (left != 1u) continue;
-            // w is ready: its record header and its other fanin's representative
-            const uint32_t y = g_y[b][t], q = g_q[b][t];
-            const uint32_t x = e_v[b][q];
-            const RecFull rw = load_full(recs, w);
-            const bool yLoad = y != x && y > (uint32_t)nPIs;
-            const unsigned long long ry = yLoad ? a.rl[y] : 0ull;
-            const uint32_t p = atomicAdd(&s_cnt[nb], 1u);
+        // when nodes and edges together outnumber the threads.  The two passes
+        // are wavefront-uniform: a wavefront reserves its ready nodes' frontier
+        // positions and edge-list ranges with one LDS atomic each (lane ranks
+        // and a prefix sum of fanout counts), instead of one per node on a
+        // single contended address.
+        const uint32_t pbase = s_pbase;
+        const uint32_t lane = tid & 63u;
+        const uint32_t t0 = (tid + WG1 - cn) % WG1;
+#pragma unroll 1
+        for (uint32_t pass = 0; pass < 2; pass++) {
+            const uint32_t t = t0 + pass * WG1;
+            const bool valid = t < ne;
+            if (__ballot(valid) == 0) continue;
+            uint32_t w = 0, yw = 0, y = 0;
+            bool ready = false;
+            if (valid) {
+                w = g_w[b][t];
+                yw = g_y[b][t];
+                y = yw & Y_ID;
+                ready = true;
+                if (y > (uint32_t)nPIs) {
+                    // two non-PI fanins: ready on the second release
+                    if (w - pbase < PB_WIN) {
+                        const uint32_t bit = 1u << (w % 32u);
+                        ready = (atomicXor(&s_pb[(w % PB_WIN) / 32], bit) & bit) != 0;
+                    } else {
+                        ready = __hip_atomic_fetch_add(&a.pending[w], 0xFFFFFFFFu, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 1u;
+                    }
+                }
+            }
+            const uint64_t rm = __ballot(ready);
+            if (rm == 0) continue;
+            // a ready node's record, overflow record and other fanin's
+            // representative, in one trip
+            uint32_t q = 0, x = 0;
+            RecFull rw{};
+            OflFull ow{};
+            unsigned long long ry = 0;
+            if (ready) {
+                q = g_q[b][t];
+                x = e_v[b][q];
+                rw = load_full(recs, w);
+                if (yw & Y_OFL) ow = load_ofl(ofls, w);
+                if (y != x && y > (uint32_t)nPIs) ry = a.rl[y];
+            }
+            const int leader = __ffsll((unsigned long long)rm) - 1;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
+            const uint32_t c = ready ? rw.cnt : 0u;
+            uint32_t incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t o = (uint32_t)__shfl_up((int)incl, off);
+                if ((int)lane >= off) incl += o;
+            }
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            uint32_t pb = 0, eb0 = 0;
+            if ((int)lane == leader) {
+                pb = atomicAdd(&s_cnt[nb], (uint32_t)__popcll((unsigned long long)rm));
+                eb0 = atomicAdd(&s_ecnt[nb], total);
+                atomicMin(&s_fmin[nb], w);       // one id per wavefront guides the counter window
+            }
+            pb = (uint32_t)__builtin_amdgcn_readlane((int)pb, leader);
+            eb0 = (uint32_t)__builtin_amdgcn_readlane((int)eb0, leader);
+            if (!ready) continue;
+            const uint32_t p = pb + rank, eb = eb0 + incl - c;
             a.order[nbase + p] = w;
             if (p >= (uint32_t)WL_FM) { s_over = 1; continue; }
             const unsigned long long sx = SRC_POS | q;
@@ -564,8 +724,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             e_s0[nb][p] = LID(rw.f0) == x ? sx : sy;
             e_s1[nb][p] = LID(rw.f1) == x ? sx : sy;
             a.rl[w] = (unsigned long long)INVALID32 | (unsigned long long)p << 32;
-            const uint32_t eb = atomicAdd(&s_ecnt[nb], rw.cnt);
-            if (eb + rw.cnt <= (uint32_t)WL_EM) put_edges(g_w[nb], g_y[nb], g_q[nb], eb, rw, (uint16_t)p, a.adj, adjY);
+            if (eb + c <= (uint32_t)WL_EM) put_edges(g_w[nb], g_y[nb], g_q[nb], eb, rw, ow, (uint16_t)p, a.adj, adjY);
             else s_over = 1;
         }
         __syncthreads();
@@ -578,11 +737,13 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         }
         const uint32_t nn = s_cnt[nb];
         const bool over = s_over != 0;
+        const uint32_t fmin = s_fmin[nb];
         if (tid == 0) {
             a.fstart[k + 1] = nbase;
             a.fcnt[k + 1] = nn;
             s_cnt[b] = 0;
             s_ecnt[b] = 0;
+            s_fmin[b] = 0xFFFFFFFFu;
         }
         last = k;
         ++k;
@@ -591,7 +752,17 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         b = nb;
         __syncthreads();
         if (over || cn == 0 || cn > thresh) break;
+        // move the counter window one step once the next frontier is a step
+        // past the margin, so that fanouts far ahead stay inside it
+        if (fmin != 0xFFFFFFFFu && fmin >= pbase + PB_MARGIN + PB_STEP) {
+            pb_release_range(s_pb, pbase, pbase + PB_STEP, a.pending, n);
+            __syncthreads();
+            pb_take_range(s_pb, pbase + PB_WIN, pbase + PB_WIN + PB_STEP, a.pending, n);
+            if (tid == 0) s_pbase = pbase + PB_STEP;
+            __syncthreads();
+        }
     }
+    pb_release_range(s_pb, s_pbase, s_pbase + PB_WIN, a.pending, n);
     if (tid == 0) {
         *kOut = k;
         if (last) atomicMax(a.depth, last);
@@ -751,6 +922,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     unsigned long long *val = t.get<unsigned long long>(n);
     uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
     NodeRec *recs = t.get<NodeRec>(n);
+    NodeOfl *ofls = t.get<NodeOfl>(n);
     SweepCtl *ctl = t.get<SweepCtl>(1);
     // room for every live node's own key plus a changed key per node
     uint64_t cap = 1024;
@@ -759,7 +931,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     unsigned long long *ht = t.get<unsigned long long>(2 * cap);
     if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
         !newId || !survIds || !survWave || !sortIds || !sortWave || !d_count || !dead || !surv || !locOf || !slotTmp || !val ||
-        !fcnt || !fstart || !d_misc || !ctl || !ht || !recs || !adjY)
+        !fcnt || !fstart || !d_misc || !ctl || !ht || !recs || !ofls || !adjY)
         return gls_fail(ctx, GLS_ERR_NOMEM, "strash: device allocation failed (n=%d)", n);
 
     GLS_HIP(ctx, hipMemsetAsync(ref, 0, sizeof(uint32_t) * n, st));
@@ -791,6 +963,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipMemsetAsync(fcnt, 0, sizeof(uint32_t) * (n + 4), st));
     GLS_HIP(ctx, hipMemsetAsync(fstart, 0, sizeof(uint32_t) * (n + 4), st));
     GLS_HIP(ctx, hipMemsetAsync(d_misc, 0, sizeof(uint32_t) * 8, st));
+    GLS_HIP(ctx, hipMemsetAsync(pending, 0, sizeof(uint32_t) * n, st));
     k_kahn_count<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cnt, pending);
     size_t tb = 0;
     GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
@@ -798,11 +971,11 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: scan storage");
     GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
-    k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cursor, adj, adjY);
+    k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cnt, cursor, adj, adjY);
     k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
     k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotTmp, &d_misc[1]);
     k_canon<<<gN, kBlock, 0, st>>>(n, nPIs, dead, ht, slotTmp, locOf);
-    k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, locOf, adjOff, adj, adjY, recs);
+    k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, locOf, adjOff, adj, adjY, recs, ofls);
     GLS_HIP(ctx, hipGetLastError());
 
     // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
@@ -815,7 +988,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
         GLS_TRY(stream_copy(ctx, &cur, &fcnt[k], sizeof(uint32_t), hipMemcpyDeviceToHost));
         if (cur == 0) break;
         if (cur <= kThresh) {
-            k_waves_1wg<<<1, WG1, 0, st>>>(k, kThresh, a, recs, adjY, &d_misc[2]);
+            k_waves_1wg<<<1, WG1, 0, st>>>(k, kThresh, a, recs, ofls, adjY, &d_misc[2], n);
             GLS_HIP(ctx, hipGetLastError());
             uint32_t kn = k;
             GLS_TRY(stream_copy(ctx, &kn, &d_misc[2], sizeof(uint32_t), hipMemcpyDeviceToHost));

@@ -3,8 +3,9 @@
 Layered AIGs whose wave widths and fanout counts cross the limits of the
 in-workgroup sweep: frontiers wider than 1024 nodes, frontiers whose fanout
 edges exceed 2048 (at a launch's first wave and in the middle of a launch),
-nodes with more fanouts than a node record holds inline (12), duplicates that
-merge within one wave and across waves, and trivially folding nodes.
+nodes with more fanouts than a node record holds inline (6), duplicates that
+merge within one wave and across waves, trivially folding nodes, and graphs
+long enough that the sweep's window of in-LDS fanout counters moves.
 """
 import sys
 
@@ -90,6 +91,10 @@ CASES = [
                                hot=(3, 6), fanin0_pool={4: 400})),
     # long narrow chain with hot nodes and folding duplicates
     ("narrow_deep", dict(seed=3, nPIs=16, widths=[20] * 300 + [900, 30] + [15] * 100, hot=(50, 150, 301))),
+    # 288k ids in narrow waves, second fanins drawn from anywhere before: the
+    # sweep's in-LDS counter window (122,880 ids) moves several times, and
+    # releases land both ahead of and behind it
+    ("window_moves", dict(seed=4, nPIs=32, widths=[24] * 12000, hot=(100, 5000, 9000))),
 ]
 
 
