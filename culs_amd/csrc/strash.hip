@@ -449,7 +449,7 @@ __device__ __forceinline__ void put_edges(uint32_t *gw, uint32_t *gy, uint16_t *
 constexpr int WG1 = 1024;
 constexpr int WL_FM = 1024;     // frontier entries
 constexpr int WL_EM = 2048;     // fanout edges of a frontier
-constexpr int WL_DT_BITS = 11;  // same-wave duplicate table: 2048 entries
+constexpr int WL_DT_BITS = 10;  // same-wave duplicate table: 1024 entries (a wave inserts at most WL_FM keys)
 constexpr int WL_DT = 1 << WL_DT_BITS;
 constexpr unsigned long long SRC_POS = 1ull << 63;
 static_assert(WL_FM <= WG1 && WL_EM <= 2 * WG1, "one node and two edges per thread");
@@ -467,6 +467,13 @@ constexpr uint32_t PB_WIN = (uint32_t)PB_WORDS * 32u;   // 122,880 ids
 constexpr uint32_t PB_STEP = PB_WIN / 4;
 constexpr uint32_t PB_MARGIN = 8192;    // ids kept below the frontier's smallest
 static_assert(PB_STEP % 32 == 0, "window steps by whole words");
+// A second, fixed bitmap holds the counters of the highest ids [tailStart, n):
+// nodes a rewrite adds are numbered after all others, so their counters would
+// otherwise never be inside the sliding window.  The sliding window stops at
+// tailStart, so no id is in both.
+constexpr int PT_WORDS = 4096;
+constexpr uint32_t PT_WIN = (uint32_t)PT_WORDS * 32u;   // 131,072 ids
+static_assert(WL_DT >= WL_FM, "every node of a wave finds a duplicate-table entry");
 
 // Source of a fanin's representative: SRC_POS | frontier position of this
 // wave, or the resolved rl value (literal | level << 32).
@@ -475,12 +482,18 @@ __device__ __forceinline__ unsigned long long src_of(unsigned long long r) {
 }
 
 // ids [lo, hi) leave the window: their set bits become one global decrement
+// (lo, hi multiples of 32; `circ` = the sliding window's circular indexing,
+// else word (id - off) / 32)
+__device__ __forceinline__ uint32_t pb_word(uint32_t base, bool circ, uint32_t off) {
+    return circ ? (base % PB_WIN) / 32 : (base - off) / 32;
+}
+
 __device__ __forceinline__ void pb_release_range(uint32_t *s_pb, uint32_t lo, uint32_t hi, uint32_t *pending,
-                                                 int32_t n) {
+                                                 int32_t n, bool circ = true, uint32_t off = 0) {
     for (uint32_t wd = lo / 32 + threadIdx.x; wd < hi / 32; wd += WG1) {
         const uint32_t base = wd * 32;
-        uint32_t bits = s_pb[(base % PB_WIN) / 32];
-        s_pb[(base % PB_WIN) / 32] = 0;
+        uint32_t bits = s_pb[pb_word(base, circ, off)];
+        s_pb[pb_word(base, circ, off)] = 0;
         while (bits) {
             const uint32_t j = __ffs(bits) - 1;
             bits &= bits - 1;
@@ -492,7 +505,8 @@ __device__ __forceinline__ void pb_release_range(uint32_t *s_pb, uint32_t lo, ui
 
 // ids [lo, hi) enter the window (their words are clear): an id with one
 // global arrival takes it over as its bit
-__device__ __forceinline__ void pb_take_range(uint32_t *s_pb, uint32_t lo, uint32_t hi, uint32_t *pending, int32_t n) {
+__device__ __forceinline__ void pb_take_range(uint32_t *s_pb, uint32_t lo, uint32_t hi, uint32_t *pending, int32_t n,
+                                              bool circ = true, uint32_t off = 0) {
     for (uint32_t wd = lo / 32 + threadIdx.x; wd < hi / 32; wd += WG1) {
         const uint32_t base = wd * 32;
         uint32_t bits = 0;
@@ -503,7 +517,7 @@ __device__ __forceinline__ void pb_take_range(uint32_t *s_pb, uint32_t lo, uint3
                 pending[w] = 2u;
             }
         }
-        s_pb[(base % PB_WIN) / 32] = bits;
+        s_pb[pb_word(base, circ, off)] = bits;
     }
 }
 
@@ -518,6 +532,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
     __shared__ unsigned long long d_key[WL_DT], d_val[WL_DT];
     __shared__ uint32_t s_cnt[2], s_ecnt[2], s_over;
     __shared__ uint32_t s_pb[PB_WORDS];
+    __shared__ uint32_t s_pt[PT_WORDS];
     __shared__ uint32_t s_pbase, s_fmin[2];
     const uint32_t tid = threadIdx.x;
     const int32_t nPIs = a.nPIs;
@@ -544,9 +559,13 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         atomicMin(&s_fmin[0], v);
     }
     __syncthreads();
-    // counter window from the frontier's smallest id
+    // counter windows: the tail of the highest ids, and a sliding window from
+    // the frontier's smallest id up to the tail
+    // rounded up to a whole word, so that [tailStart, n) spans at most PT_WORDS words
+    const uint32_t tailStart = (((uint32_t)n > PT_WIN ? (uint32_t)n - PT_WIN : 0u) + 31u) & ~31u;
+    pb_take_range(s_pt, tailStart, (uint32_t)n + 31u, a.pending, n, false, tailStart);
     const uint32_t pb0 = (s_fmin[0] > PB_MARGIN ? s_fmin[0] - PB_MARGIN : 0u) & ~31u;
-    pb_take_range(s_pb, pb0, pb0 + PB_WIN, a.pending, n);
+    pb_take_range(s_pb, pb0, min(pb0 + PB_WIN, tailStart), a.pending, n);
     if (tid == 0) { s_pbase = pb0; s_fmin[0] = 0xFFFFFFFFu; }
     __syncthreads();
     for (uint32_t i = tid; i < cn; i += WG1) {
@@ -567,7 +586,8 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
     }
     __syncthreads();
     if (s_over) {
-        pb_release_range(s_pb, s_pbase, s_pbase + PB_WIN, a.pending, n);
+        pb_release_range(s_pb, s_pbase, min(s_pbase + PB_WIN, tailStart), a.pending, n);
+        pb_release_range(s_pt, tailStart, (uint32_t)n + 31u, a.pending, n, false, tailStart);
         if (tid == 0) *kOut = k;
         return;
     }
@@ -668,8 +688,10 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
                 ready = true;
                 if (y > (uint32_t)nPIs) {
                     // two non-PI fanins: ready on the second release
-                    if (w - pbase < PB_WIN) {
-                        const uint32_t bit = 1u << (w % 32u);
+                    const uint32_t bit = 1u << (w % 32u);
+                    if (w >= tailStart) {
+                        ready = (atomicXor(&s_pt[(w - tailStart) / 32], bit) & bit) != 0;
+                    } else if (w - pbase < PB_WIN) {
                         ready = (atomicXor(&s_pb[(w % PB_WIN) / 32], bit) & bit) != 0;
                     } else {
                         ready = __hip_atomic_fetch_add(&a.pending[w], 0xFFFFFFFFu, __ATOMIC_RELAXED,
@@ -696,12 +718,15 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
             const uint32_t c = ready ? rw.cnt : 0u;
+            // inclusive prefix sum over the wavefront: shifts within rows of 16
+            // lanes, then each row adds the totals of the rows before it
             uint32_t incl = c;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t o = (uint32_t)__shfl_up((int)incl, off);
-                if ((int)lane >= off) incl += o;
-            }
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xF, 0xF, false);   // row_shr:1
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xF, 0xF, false);   // row_shr:2
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xF, 0xF, false);   // row_shr:4
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xF, 0xF, false);   // row_shr:8
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x142, 0xA, 0xF, false);   // row_bcast:15
+            incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, 0x143, 0xC, 0xF, false);   // row_bcast:31
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             uint32_t pb = 0, eb0 = 0;
             if ((int)lane == leader) {
@@ -755,14 +780,15 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         // move the counter window one step once the next frontier is a step
         // past the margin, so that fanouts far ahead stay inside it
         if (fmin != 0xFFFFFFFFu && fmin >= pbase + PB_MARGIN + PB_STEP) {
-            pb_release_range(s_pb, pbase, pbase + PB_STEP, a.pending, n);
+            pb_release_range(s_pb, pbase, min(pbase + PB_STEP, tailStart), a.pending, n);
             __syncthreads();
-            pb_take_range(s_pb, pbase + PB_WIN, pbase + PB_WIN + PB_STEP, a.pending, n);
+            pb_take_range(s_pb, min(pbase + PB_WIN, tailStart), min(pbase + PB_WIN + PB_STEP, tailStart), a.pending, n);
             if (tid == 0) s_pbase = pbase + PB_STEP;
             __syncthreads();
         }
     }
-    pb_release_range(s_pb, s_pbase, s_pbase + PB_WIN, a.pending, n);
+    pb_release_range(s_pb, s_pbase, min(s_pbase + PB_WIN, tailStart), a.pending, n);
+    pb_release_range(s_pt, tailStart, (uint32_t)n + 31u, a.pending, n, false, tailStart);
     if (tid == 0) {
         *kOut = k;
         if (last) atomicMax(a.depth, last);
