@@ -276,18 +276,28 @@ __device__ __forceinline__ uint32_t min8_lane0(uint32_t v) {
 struct Slot {
     uint32_t l0, l1, l2, l3, tt, value;
     int n, used;
+    uint32_t sig;       // bit (leaf & 31) of every leaf: a prefilter for containment
 };
+
+__device__ __forceinline__ uint32_t sig4(uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3, int nn) {
+    return (nn > 0 ? 1u << (n0 & 31) : 0u) | (nn > 1 ? 1u << (n1 & 31) : 0u) | (nn > 2 ? 1u << (n2 & 31) : 0u) |
+           (nn > 3 ? 1u << (n3 & 31) : 0u);
+}
 
 // CutFilter (rewrite.cu:178-193) of the cut in slot `id` (leaves N, nn) against
 // the other used slots.  Scanning slots in order, a larger slot that contains N
 // is dropped; the first slot of no more leaves that N contains drops N and ends
 // the scan.  Returns true if N was dropped.
 __device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t n0, uint32_t n1, uint32_t n2,
-                                            uint32_t n3, int nn) {
+                                            uint32_t n3, int nn, uint32_t nsig) {
     const bool mine = lane < NCUT && lane != id && s.used;
     const bool bigger = s.n > nn;
+    // containment implies containment of the leaf signatures (CutFilter's own
+    // prefilter, rewrite.cu:181, 185): when no slot passes it, nothing changes
+    const bool maybe = mine && (bigger ? (s.sig & nsig) == nsig : (s.sig & nsig) == s.sig);
+    if (__ballot(maybe) == 0) return false;
     bool d1 = false, d2 = false;
-    if (mine) {
+    if (maybe) {
         if (bigger) {
             d1 = (nn < 1 || leaf_in(n0, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 2 || leaf_in(n1, s.l0, s.l1, s.l2, s.l3, s.n)) &&
                  (nn < 3 || leaf_in(n2, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 4 || leaf_in(n3, s.l0, s.l1, s.l2, s.l3, s.n));
@@ -402,6 +412,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     s.tt = lane == 0 ? 0xAAAAu : 0u;
     s.value = lane == 0 ? 1001u : 0u;
     s.used = lane == 0 ? 1 : 0;
+    s.sig = lane == 0 ? 1u << ((uint32_t)idx & 31) : 0u;
     uint64_t todo = __ballot(status != C_SKIP);
     while (todo) {
         const int l = __ffsll((unsigned long long)todo) - 1;
@@ -424,8 +435,9 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         const int nn = rl(m.nLeaves, l);
         const uint32_t n0 = (uint32_t)rl((int)m.leaves[0], l), n1 = (uint32_t)rl((int)m.leaves[1], l);
         const uint32_t n2 = (uint32_t)rl((int)m.leaves[2], l), n3 = (uint32_t)rl((int)m.leaves[3], l);
-        if (lane == id) { s.l0 = n0; s.l1 = n1; s.l2 = n2; s.l3 = n3; s.n = nn; s.used = 1; }
-        if (filter_wave(s, lane, id, n0, n1, n2, n3, nn)) continue;
+        const uint32_t nsig = sig4(n0, n1, n2, n3, nn);
+        if (lane == id) { s.l0 = n0; s.l1 = n1; s.l2 = n2; s.l3 = n3; s.n = nn; s.used = 1; s.sig = nsig; }
+        if (filter_wave(s, lane, id, n0, n1, n2, n3, nn, nsig)) continue;
         const uint32_t ttv = (uint32_t)rl((int)m.tt, l);
         if (lane == id) s.tt = ttv;
         int fn = nn;
@@ -434,8 +446,9 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
             const uint32_t q0 = (uint32_t)rl((int)mm.leaves[0], l), q1 = (uint32_t)rl((int)mm.leaves[1], l);
             const uint32_t q2 = (uint32_t)rl((int)mm.leaves[2], l), q3 = (uint32_t)rl((int)mm.leaves[3], l);
             const uint32_t qt = (uint32_t)rl((int)mm.tt, l);
-            if (lane == id) { s.l0 = q0; s.l1 = q1; s.l2 = q2; s.l3 = q3; s.n = fn; s.tt = qt; }
-            filter_wave(s, lane, id, q0, q1, q2, q3, fn);
+            const uint32_t qsig = sig4(q0, q1, q2, q3, fn);
+            if (lane == id) { s.l0 = q0; s.l1 = q1; s.l2 = q2; s.l3 = q3; s.n = fn; s.tt = qt; s.sig = qsig; }
+            filter_wave(s, lane, id, q0, q1, q2, q3, fn, qsig);
         }
         const uint32_t val = (uint32_t)rl((int)mm.value, l);
         if (lane == id) s.value = val;
