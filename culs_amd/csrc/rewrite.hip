@@ -404,6 +404,13 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
             mm.value = cut_value(mm, nRef);
         }
     }
+    // per-candidate words read by the replay: status, shrink flag, leaf counts
+    // and value in one; both truth tables in one; both leaf signatures
+    const uint32_t info = (uint32_t)status | (uint32_t)shrunk << 2 | (uint32_t)(m.nLeaves & 7) << 3 |
+                          (uint32_t)(mm.nLeaves & 7) << 6 | (mm.value & 0x7FFu) << 9;
+    const uint32_t tts = (m.tt & 0xFFFFu) | (mm.tt & 0xFFFFu) << 16;
+    const uint32_t msig = sig4(m.leaves[0], m.leaves[1], m.leaves[2], m.leaves[3], m.nLeaves);
+    const uint32_t mmsig = sig4(mm.leaves[0], mm.leaves[1], mm.leaves[2], mm.leaves[3], mm.nLeaves);
     // ---- phase 2: replay the insertions in (i, j) order
     Slot s;
     s.l0 = lane == 0 ? (uint32_t)idx : 0u;
@@ -417,7 +424,8 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     while (todo) {
         const int l = __ffsll((unsigned long long)todo) - 1;
         todo &= todo - 1;
-        const int st = rl(status, l);
+        const uint32_t inf = (uint32_t)rl((int)info, l);
+        const int st = (int)(inf & 3u);
         // FindCut (rewrite.cu:84-109)
         const uint64_t usedMask = __ballot(lane < NCUT && s.used) & 0xFFull;
         int id;
@@ -432,25 +440,25 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         }
         if (lane == id) s.used = 0;
         if (st == C_FAIL) continue;
-        const int nn = rl(m.nLeaves, l);
+        const int nn = (int)((inf >> 3) & 7u);
         const uint32_t n0 = (uint32_t)rl((int)m.leaves[0], l), n1 = (uint32_t)rl((int)m.leaves[1], l);
         const uint32_t n2 = (uint32_t)rl((int)m.leaves[2], l), n3 = (uint32_t)rl((int)m.leaves[3], l);
-        const uint32_t nsig = sig4(n0, n1, n2, n3, nn);
+        const uint32_t nsig = (uint32_t)rl((int)msig, l);
         if (lane == id) { s.l0 = n0; s.l1 = n1; s.l2 = n2; s.l3 = n3; s.n = nn; s.used = 1; s.sig = nsig; }
         if (filter_wave(s, lane, id, n0, n1, n2, n3, nn, nsig)) continue;
-        const uint32_t ttv = (uint32_t)rl((int)m.tt, l);
-        if (lane == id) s.tt = ttv;
+        const uint32_t ttw = (uint32_t)rl((int)tts, l);
+        if (lane == id) s.tt = ttw & 0xFFFFu;
         int fn = nn;
-        if (rl(shrunk, l)) {
-            fn = rl(mm.nLeaves, l);
+        if ((inf >> 2) & 1u) {
+            fn = (int)((inf >> 6) & 7u);
             const uint32_t q0 = (uint32_t)rl((int)mm.leaves[0], l), q1 = (uint32_t)rl((int)mm.leaves[1], l);
             const uint32_t q2 = (uint32_t)rl((int)mm.leaves[2], l), q3 = (uint32_t)rl((int)mm.leaves[3], l);
-            const uint32_t qt = (uint32_t)rl((int)mm.tt, l);
-            const uint32_t qsig = sig4(q0, q1, q2, q3, fn);
+            const uint32_t qt = ttw >> 16;
+            const uint32_t qsig = (uint32_t)rl((int)mmsig, l);
             if (lane == id) { s.l0 = q0; s.l1 = q1; s.l2 = q2; s.l3 = q3; s.n = fn; s.tt = qt; s.sig = qsig; }
             filter_wave(s, lane, id, q0, q1, q2, q3, fn, qsig);
         }
-        const uint32_t val = (uint32_t)rl((int)mm.value, l);
+        const uint32_t val = (inf >> 9) & 0x7FFu;
         if (lane == id) s.value = val;
         if (fn < 2) break;
     }
