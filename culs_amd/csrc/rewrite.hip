@@ -660,10 +660,11 @@ struct GraphView {
 };
 
 // per-thread column of an LDS array laid out [j][lane]
-struct Col {
-    int32_t *p;
-    __device__ __forceinline__ int32_t &operator[](int j) const { return p[j * EVAL_BLOCK]; }
+template <typename T> struct ColT {
+    T *p;
+    __device__ __forceinline__ T &operator[](int j) const { return p[j * EVAL_BLOCK]; }
 };
+using Col = ColT<int32_t>;
 
 struct Mffc {
     int size;
@@ -725,17 +726,21 @@ __device__ __forceinline__ int table_lookup(const GraphView &g, int in0, int in1
 }
 
 // library matching of one cut (rewrite.cu:391-418); returns the class
+// LV: storage of the structure nodes' levels, uint16_t when every level is
+// below 65535 (more workgroups fit a CU's LDS), else int32_t; the values are
+// the same either way.
+template <typename LV>
 __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, const int *leaves,
-                         const Mffc &t, Col match, Col mlev, uint32_t &isC) {
+                         const Mffc &t, Col match, ColT<LV> mlev, uint32_t &isC) {
     uint32_t w = L.npn[tt];
     int uPhase = w & 31, perm = (w >> 5) & 31, cls = (w >> 10) & 255;
     const uint8_t *pp = L.perms4 + 4 * perm;
-    for (int j = 0; j < NMATCH; j++) { match[j] = -1; mlev[j] = -1; }
+    for (int j = 0; j < NMATCH; j++) { match[j] = -1; mlev[j] = (LV)-1; }
     isC = 0;
     for (int j = 0; j < 4; j++) {
         int leaf = leaves[pp[j]];
         match[j] = leaf;
-        mlev[j] = (int32_t)g.level[leaf];
+        mlev[j] = (LV)g.level[leaf];
         if ((uPhase >> j) & 1) isC |= 1u << j;
     }
     const int nn = lib_nn(L, cls);
@@ -743,14 +748,14 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
         uint32_t wd = lib_word(L, cls, j);
         int a = wd & 63, b = (wd >> 6) & 63, ca = (wd >> 12) & 1, cb = (wd >> 13) & 1;
         int num = j + 4;
-        int la = mlev[a], lb = mlev[b];
-        mlev[num] = 1 + (la > lb ? la : lb);
+        const int la = (int)mlev[a], lb = (int)mlev[b];
+        mlev[num] = (LV)(1 + (la > lb ? la : lb));
         int ma = match[a], mb = match[b];
         if (ma == -1 || mb == -1 || ma == id || mb == id) continue;
         int nid = table_lookup(g, ma, mb, (int)isc_bit(isC, a) ^ ca, (int)isc_bit(isC, b) ^ cb);
         if (nid != -1 && !mffc_deleted(t, nid)) {
             match[num] = nid;
-            mlev[num] = (int32_t)g.level[nid];
+            mlev[num] = (LV)g.level[nid];
         }
     }
     return cls;
@@ -776,13 +781,16 @@ __device__ __forceinline__ void cut_leaves4(const uint32_t *p, int *leaves, int 
     if (nL == 3) { leaves[3] = 0; nL = 4; }
 }
 
+template <typename LV>
 __global__ void __launch_bounds__(EVAL_BLOCK) k_eval(int32_t n, int32_t nPIs, GraphView g, Lib L,
                                                       const uint32_t *__restrict__ cuts, int useZeros,
                                                       int4 *best) {
-    __shared__ int32_t s_match[NMATCH * EVAL_BLOCK], s_mlev[NMATCH * EVAL_BLOCK];
+    __shared__ int32_t s_match[NMATCH * EVAL_BLOCK];
+    __shared__ LV s_mlev[NMATCH * EVAL_BLOCK];
     int64_t v = nPIs + 1 + blockIdx.x * (int64_t)EVAL_BLOCK + threadIdx.x;
     if (v >= n) return;
-    Col match{s_match + threadIdx.x}, mlev{s_mlev + threadIdx.x};
+    Col match{s_match + threadIdx.x};
+    ColT<LV> mlev{s_mlev + threadIdx.x};
     const int id = (int)v;
     int reduction = -1, bestLevel = 99999999, bestCut = -1, bestOut = 0;
     const uint32_t *cs = cuts + (size_t)v * NCUT * 5;
@@ -804,7 +812,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_eval(int32_t n, int32_t nPIs, Gr
             int added = __popcll(added_set(L, cls, rt, match));
             int gain = saved - added;
             if (gain < 0 || (gain == 0 && !useZeros)) continue;
-            int rl = mlev[rt];
+            int rl = (int)mlev[rt];
             if (gain < reduction || (gain == reduction && rl >= bestLevel)) continue;
             reduction = gain;
             bestLevel = rl;
@@ -1465,7 +1473,7 @@ int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const 
     std::vector<int4> none(g.nPIs + 1, make_int4(-1, 0, 0, 0));
     GLS_HIP(ctx, hipMemcpyAsync(d_best, none.data(), sizeof(int4) * none.size(), hipMemcpyHostToDevice, st));
     if (nAnd > 0)
-        k_eval<<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(g.nObjs, g.nPIs, gv, lib_of(ctx), d_cuts, useZeros,
+        k_eval<int32_t><<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(g.nObjs, g.nPIs, gv, lib_of(ctx), d_cuts, useZeros,
                                                                 reinterpret_cast<int4 *>(d_best));
     GLS_HIP(ctx, hipGetLastError());
     GLS_HIP(ctx, hipStreamSynchronize(st));
@@ -1519,8 +1527,12 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         GLS_HIP(ctx, hipMemcpyAsync(best, none.data(), sizeof(int4) * none.size(), hipMemcpyHostToDevice, st));
         const int32_t nAnd = n - nPIs - 1;
         const Lib L = lib_of(ctx);
-        if (nAnd > 0)
-            k_eval<<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, nPIs, gv, L, cuts, useZeros, best);
+        // a structure node's level exceeds the graph's deepest by at most
+        // NMATCH, so below 65000 levels every value fits 16 bits under 0xFFFF
+        if (nAnd > 0 && levelStart.size() < 65000)
+            k_eval<uint16_t><<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, nPIs, gv, L, cuts, useZeros, best);
+        else if (nAnd > 0)
+            k_eval<int32_t><<<blocks(nAnd, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, nPIs, gv, L, cuts, useZeros, best);
         GLS_HIP(ctx, hipGetLastError());
         tm.mark(); // 4
         // candidates in id order
