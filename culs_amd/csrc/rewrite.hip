@@ -671,36 +671,61 @@ struct Mffc {
     int32_t id[MFFC_TAB], num[MFFC_TAB];
 };
 
+// The table and the DFS stack below are indexed by constants only (a write at
+// a run-time position selects among the entries), so they live in registers
+// instead of scratch memory.
 __device__ __forceinline__ int mffc_decrease(const GraphView &g, int x, const int *leaves, int nL, Mffc &t) {
+#pragma unroll
     for (int i = 0; i < 4; i++)
         if (i < nL && x == leaves[i]) return 1;
+    int found = -1;
+#pragma unroll
     for (int i = 0; i < MFFC_TAB; i++)
-        if (i < t.size && t.id[i] == x) return --t.num[i];
+        if (found < 0 && i < t.size && t.id[i] == x) found = i;
+    if (found >= 0) {
+        int r = 0;
+#pragma unroll
+        for (int i = 0; i < MFFC_TAB; i++)
+            if (i == found) r = --t.num[i];
+        return r;
+    }
     if (t.size == MFFC_TAB) return 1;
-    int r = g.nRef[x] - 1;
-    t.id[t.size] = x;
-    t.num[t.size] = r;
+    const int r = g.nRef[x] - 1;
+#pragma unroll
+    for (int i = 0; i < MFFC_TAB; i++)
+        if (i == t.size) { t.id[i] = x; t.num[i] = r; }
     t.size++;
     return r;
 }
 
 // CalcMFFC (rewrite.cu:319-326) without recursion: an explicit DFS stack that
 // visits fanin0's subtree before decreasing fanin1, as the recursion does.
+// At most MFFC_TAB nodes are pushed besides the root (a push needs a new
+// table entry), so MFFC_TAB + 2 entries never overflow.
 __device__ int calc_mffc(const GraphView &g, int root, const int *leaves, int nL, Mffc &t) {
-    int stkNode[MFFC_TAB + 2], stkStage[MFFC_TAB + 2];
-    int sp = 1, ans = 1;
+    constexpr int S = MFFC_TAB + 2;
+    int stkNode[S], stkStage[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) { stkNode[j] = 0; stkStage[j] = 0; }
     stkNode[0] = root;
-    stkStage[0] = 0;
+    int sp = 1, ans = 1;
     t.size = 0;
+#pragma unroll
+    for (int i = 0; i < MFFC_TAB; i++) { t.id[i] = 0; t.num[i] = 0; }
     while (sp) {
-        int cur = stkNode[sp - 1];
-        int stage = stkStage[sp - 1];
+        int cur = 0, stage = 0;
+#pragma unroll
+        for (int j = 0; j < S; j++)
+            if (j == sp - 1) { cur = stkNode[j]; stage = stkStage[j]; }
         if (stage < 2) {
-            stkStage[sp - 1] = stage + 1;
-            int x = (int)LID(stage == 0 ? g.f0[cur] : g.f1[cur]);
+#pragma unroll
+            for (int j = 0; j < S; j++)
+                if (j == sp - 1) stkStage[j] = stage + 1;
+            const int x = (int)LID(stage == 0 ? g.f0[cur] : g.f1[cur]);
             if (mffc_decrease(g, x, leaves, nL, t) == 0) {
-                stkNode[sp] = x;
-                stkStage[sp] = 0;
+#pragma unroll
+                for (int j = 0; j < S; j++)
+                    if (j == sp) { stkNode[j] = x; stkStage[j] = 0; }
                 sp++;
                 ans++;
             }
@@ -712,9 +737,15 @@ __device__ int calc_mffc(const GraphView &g, int root, const int *leaves, int nL
 }
 
 __device__ __forceinline__ int mffc_deleted(const Mffc &t, int x) {
+    int r = 0;
+    bool found = false;
+#pragma unroll
     for (int i = 0; i < MFFC_TAB; i++)
-        if (i < t.size && t.id[i] == x) return t.num[i] == 0;
-    return 0;
+        if (!found && i < t.size && t.id[i] == x) {
+            found = true;
+            r = t.num[i] == 0;
+        }
+    return r;
 }
 
 __device__ __forceinline__ int table_lookup(const GraphView &g, int in0, int in1, int c0, int c1) {
@@ -738,7 +769,8 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
     for (int j = 0; j < NMATCH; j++) { match[j] = -1; mlev[j] = (LV)-1; }
     isC = 0;
     for (int j = 0; j < 4; j++) {
-        int leaf = leaves[pp[j]];
+        const int q = pp[j];
+        const int leaf = q == 0 ? leaves[0] : q == 1 ? leaves[1] : q == 2 ? leaves[2] : leaves[3];
         match[j] = leaf;
         mlev[j] = (LV)g.level[leaf];
         if ((uPhase >> j) & 1) isC |= 1u << j;
