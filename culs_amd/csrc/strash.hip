@@ -665,15 +665,17 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
             }
         }
         // ---- phase A: release the fanout edges.  Edge t runs on thread
-        // (t + cn) mod WG1, so the threads hashing a node take edges only
-        // when nodes and edges together outnumber the threads.  The two passes
+        // (t + cn rounded up to a wavefront) mod WG1: no wavefront both hashes
+        // a node (and waits for its table atomic) and releases edges, unless
+        // nodes and edges together outnumber the threads.  The two passes
         // are wavefront-uniform: a wavefront reserves its ready nodes' frontier
         // positions and edge-list ranges with one LDS atomic each (lane ranks
         // and a prefix sum of fanout counts), instead of one per node on a
         // single contended address.
         const uint32_t pbase = s_pbase;
         const uint32_t lane = tid & 63u;
-        const uint32_t t0 = (tid + WG1 - cn) % WG1;
+        const uint32_t cnw = (cn + 63u) & ~63u;
+        const uint32_t t0 = (tid + WG1 - cnw) % WG1;
 #pragma unroll 1
         for (uint32_t pass = 0; pass < 2; pass++) {
             const uint32_t t = t0 + pass * WG1;
