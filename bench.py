@@ -13,6 +13,7 @@ Also printed in the same JSON line:
   qor           AND counts, depth, and a simulation equivalence check.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nands N] [--cpu-sample N] [--cut-mode M]
+       [--stage rw|cuts|strash] [--workload rw10m|batch|serve]
 """
 import argparse
 import json
@@ -59,8 +60,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
                     help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
-    ap.add_argument("--stage", choices=["rw", "cuts"], default="rw",
-                    help="cuts: time only reorder + cut enumeration (gls_enumerate_cuts), for counter runs")
+    ap.add_argument("--stage", choices=["rw", "cuts", "strash"], default="rw",
+                    help="cuts: time only reorder + cut enumeration (gls_enumerate_cuts); strash: only the "
+                         "strash of the generated input; both for counter runs")
     ap.add_argument("--workload", choices=["rw10m", "batch", "serve"], default="rw10m",
                     help="rw10m: the headline line; batch: 4096 mixed AIGs packed into one arena vs one at a "
                          "time; serve: a Poisson stream of `b; rw; rwz; b` jobs, served vs a per-command loop")
@@ -105,6 +107,8 @@ def main():
     s = src.stats()
     n_in = s.nAnds
     log("strash: %d -> %d ANDs, depth %d, %.1f ms" % (a.nands, n_in, s.levels, st_info["total_ms"]))
+    if a.stage == "strash":
+        return
 
     def step():
         c = src.copy()
