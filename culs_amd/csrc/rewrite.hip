@@ -108,57 +108,41 @@ __device__ __forceinline__ void unpack_cut(const uint32_t *p, DCut &c) {
     c.used = (m >> 30) & 1;
 }
 
-// Sorted union of two sorted leaf sets (rewrite.cu:112-163); fails above 4
-// leaves.  Each leaf goes to its rank in the union, computed by compares, and
-// every output slot selects its leaf from the eight candidates, so the leaf
-// arrays are indexed by constants only and stay in registers.
+// Sorted union of two sorted leaf sets (rewrite.cu:112-163), the reference's
+// two-pointer walk unrolled over the four output positions; fails above 4
+// leaves.  The current element of each set is selected with constant indexes,
+// so the leaf arrays stay in registers.
+__device__ __forceinline__ uint32_t sel4(const uint32_t *l, int i) {
+    return i == 0 ? l[0] : i == 1 ? l[1] : i == 2 ? l[2] : l[3];
+}
+
 __device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut &c) {
-    bool nb[4];    // leaf j of b is new: valid and not a leaf of a
-    int ra[4], rb[4], n = a.nLeaves;
+    int i = 0, k = 0, n = 0;
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        bool d = false;
-#pragma unroll
-        for (int i = 0; i < 4; i++) d |= (i < a.nLeaves && a.leaves[i] == b.leaves[j]);
-        nb[j] = j < b.nLeaves && !d;
-        n += nb[j];
+    for (int s = 0; s < 4; s++) {
+        const bool ha = i < a.nLeaves, hb = k < b.nLeaves;
+        const uint32_t ai = sel4(a.leaves, i < 3 ? i : 3), bk = sel4(b.leaves, k < 3 ? k : 3);
+        const bool takeA = ha && (!hb || ai <= bk);
+        const bool takeB = hb && (!ha || bk <= ai);
+        if (ha || hb) o[s] = takeA ? ai : bk;
+        n += (ha || hb);
+        i += takeA;
+        k += takeB;
     }
-    if (n > 4) return 0;
-    // rank in the union = smaller leaves of a + smaller new leaves of b
+    if (i < a.nLeaves || k < b.nLeaves) return 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int r = i;
-#pragma unroll
-        for (int j = 0; j < 4; j++) r += (nb[j] && b.leaves[j] < a.leaves[i]);
-        ra[i] = i < a.nLeaves ? r : 8;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int r = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) r += (i < a.nLeaves && a.leaves[i] < b.leaves[j]);
-#pragma unroll
-        for (int k = 0; k < j; k++) r += nb[k];
-        rb[j] = nb[j] ? r : 8;
-    }
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        uint32_t x = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) x = ra[i] == p ? a.leaves[i] : x;
-#pragma unroll
-        for (int j = 0; j < 4; j++) x = rb[j] == p ? b.leaves[j] : x;
-        c.leaves[p] = x;
-    }
+    for (int p = 0; p < 4; p++) c.leaves[p] = o[p];
     c.nLeaves = n;
     return 1;
 }
 
+// swaps truth-table variables v and v + 1 (v in 0..2), without branches
 __device__ __forceinline__ uint32_t swap_adj(uint32_t t, int v) {
-    if (v == 0) return (t & 0x99999999u) | ((t & 0x22222222u) << 1) | ((t & 0x44444444u) >> 1);
-    if (v == 1) return (t & 0xC3C3C3C3u) | ((t & 0x0C0C0C0Cu) << 2) | ((t & 0x30303030u) >> 2);
-    if (v == 2) return (t & 0xF00FF00Fu) | ((t & 0x00F000F0u) << 4) | ((t & 0x0F000F00u) >> 4);
-    return 0;
+    const uint32_t up = v == 0 ? 0x22222222u : v == 1 ? 0x0C0C0C0Cu : 0x00F000F0u;
+    const uint32_t sh = 1u << v;
+    const uint32_t dn = up << sh;
+    return (t & ~(up | dn)) | ((t & up) << sh) | ((t & dn) >> sh);
 }
 
 __device__ uint32_t tt_stretch(uint32_t t, int nVar, int phase) {
