@@ -116,8 +116,12 @@ __device__ __forceinline__ uint32_t sel4(const uint32_t *l, int i) {
     return i == 0 ? l[0] : i == 1 ? l[1] : i == 2 ? l[2] : l[3];
 }
 
-__device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut &c) {
+// phA / phB: the output positions that hold a's / b's leaves (the truth-table
+// phases of rewrite.cu:195-201, read off the walk itself)
+__device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut &c, int &phA, int &phB) {
     int i = 0, k = 0, n = 0;
+    phA = 0;
+    phB = 0;
     uint32_t o[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int s = 0; s < 4; s++) {
@@ -126,6 +130,8 @@ __device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut 
         const bool takeA = ha && (!hb || ai <= bk);
         const bool takeB = hb && (!ha || bk <= ai);
         if (ha || hb) o[s] = takeA ? ai : bk;
+        phA |= takeA ? 1 << s : 0;
+        phB |= takeB ? 1 << s : 0;
         n += (ha || hb);
         i += takeA;
         k += takeB;
@@ -153,16 +159,6 @@ __device__ uint32_t tt_stretch(uint32_t t, int nVar, int phase) {
             var--;
         }
     return t;
-}
-
-__device__ __forceinline__ int truth_phase(const DCut &x, const DCut &c) {
-    int ph = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (i < c.nLeaves && j < x.nLeaves && x.leaves[j] == c.leaves[i]) ph |= 1 << i;
-    return ph;
 }
 
 __device__ uint32_t tt_shrink(uint32_t t, int phase) {
@@ -374,14 +370,15 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     m.nLeaves = 0; m.tt = 0; m.value = 0; m.used = 0;
     mm = m;
     if (a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4) {
-        const int ok = merge_ordered(a, b, m);
+        int phA, phB;
+        const int ok = merge_ordered(a, b, m, phA, phB);
         if (!ok) {
             status = C_FAIL;
         } else {
             status = C_OK;
             uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
-            t0 = tt_stretch(t0, a.nLeaves, truth_phase(a, m));
-            t1 = tt_stretch(t1, b.nLeaves, truth_phase(b, m));
+            t0 = tt_stretch(t0, a.nLeaves, phA);
+            t1 = tt_stretch(t1, b.nLeaves, phB);
             m.tt = (t0 & t1) & 0xFFFF;
             mm = m;
             shrunk = minimize_support(mm);
