@@ -37,6 +37,23 @@ B_CUT_PER_AND = 3 * K_CUTS * CUT_BYTES + 12
 TRAFFIC_BYTES_PER_LAUNCH = None
 
 
+def cut_kernel_name(mode):
+    """The cut-enumeration kernel that a rewrite with this schedule launches."""
+    import culs_amd
+    return {culs_amd.GLS_RW_XCD_CUTS: "k_cuts_xcd", culs_amd.GLS_RW_LEVEL_CUTS: "k_cuts_level",
+            culs_amd.GLS_RW_COOP_CUTS: "k_cuts_levels", culs_amd.GLS_RW_DATAFLOW_CUTS: "k_cuts_dataflow",
+            culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair"}.get(mode, "k_cuts_triple")
+
+
+def cut_launches(mode, nlev):
+    """Cut-enumeration launches per rw pass over nlev AND levels."""
+    import culs_amd
+    if mode in (culs_amd.GLS_RW_XCD_CUTS, culs_amd.GLS_RW_COOP_CUTS, culs_amd.GLS_RW_DATAFLOW_CUTS):
+        return 1
+    per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2}.get(mode, 3)
+    return (nlev + per - 1) // per
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -148,10 +165,10 @@ def main():
     n_out = out.stats().nAnds
     depth_out = out.stats().levels
 
-    # dominant kernel: the per-level cut enumeration kernel (k_cuts_level).  Every
-    # 64 levels a window of 8 consecutive launches is timed with HIP events on
-    # the first and last dispatch (the context stream), so the average is the
-    # per-launch kernel time as rocprofv3 reports it.
+    # dominant kernel: the cut enumeration kernel (k_cuts_triple: three levels
+    # per launch).  Every 64 levels a window of 8 consecutive launches is timed with
+    # HIP events on the first and last dispatch (the context stream), so the
+    # average is the per-launch kernel time as rocprofv3 reports it.
     ks = sum(i["kernel_samples"] for i in infos)
     kms = sum(i["kernel_sample_ms"] for i in infos)
     knodes = sum(i["kernel_sample_nodes"] for i in infos)
@@ -164,8 +181,8 @@ def main():
         avg_launch_s, bytes_per_launch, achieved = 0.0, n_in * B_CUT_PER_AND / nlev, 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                "kernel": "k_cuts_xcd" if a.cut_mode == culs_amd.GLS_RW_XCD_CUTS else "k_cuts_level",
-                "launches_per_step": 1 if a.cut_mode == culs_amd.GLS_RW_XCD_CUTS else nlev, "timed_launches": ks,
+                "kernel": cut_kernel_name(a.cut_mode), "launches_per_step": cut_launches(a.cut_mode, nlev),
+                "timed_launches": ks,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
                 "bytes_per_node": B_CUT_PER_AND,
                 "pass_alg_GBs": round(n_in * B_RW_PER_AND / (total / a.steps) / 1e9, 3)}

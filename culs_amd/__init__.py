@@ -19,6 +19,8 @@ GLS_ST_CPU = 1
 GLS_RW_COOP_CUTS = 2
 GLS_RW_DATAFLOW_CUTS = 4
 GLS_RW_XCD_CUTS = 8
+GLS_RW_LEVEL_CUTS = 16
+GLS_RW_PAIR_CUTS = 32
 
 
 class GlsError(RuntimeError):
@@ -259,8 +261,9 @@ class Aig:
 
     def rewrite(self, use_zeros=False, cut_mode=0):
         """One rw pass (rw -z / rwz with use_zeros).  cut_mode schedules cut
-        enumeration: 0 one launch per level, GLS_RW_COOP_CUTS one cooperative
-        launch, GLS_RW_DATAFLOW_CUTS fanin-driven, GLS_RW_XCD_CUTS one persistent
+        enumeration: 0 one launch per three levels, GLS_RW_PAIR_CUTS per two,
+        GLS_RW_LEVEL_CUTS per level, GLS_RW_COOP_CUTS one cooperative launch,
+        GLS_RW_DATAFLOW_CUTS fanin-driven, GLS_RW_XCD_CUTS one persistent
         kernel on one XCD; the results are identical."""
         flags = cut_mode
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")

@@ -332,17 +332,17 @@ __device__ __forceinline__ void store_tagged(unsigned long long *rec, uint32_t l
 // cut_node record modes
 enum : int { REC_PLAIN = 0, REC_TAGGED = 1 };
 
-// Cut set of node idx, computed by one whole wavefront (lane = thread lane).
-// MODE REC_TAGGED: fanin cut sets are polled in the fence-free format and the
-// result is written in it (tcuts, 24 bytes per slot); REC_PLAIN: plain 20-byte
-// records.
+// Cut set of node idx (fanin literals l0, l1), computed by one whole wavefront
+// (lane = thread lane).  MODE REC_PLAIN: the fanins' 8-slot records (20 bytes
+// per slot) start at ra and rb, in global memory or LDS, and the result goes to
+// out in the same format.  MODE REC_TAGGED: fanin cut sets are polled in the
+// fence-free format and the result is written in it (tcuts, 24 bytes per slot).
 // Returns false if a poll exceeded its bound (*err set), true otherwise.
 template <int MODE>
-__device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t *__restrict__ f0,
-                                         const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
-                                         uint32_t *cuts, unsigned long long *tcuts, uint32_t *err,
+__device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t l0, const uint32_t l1,
+                                         const uint32_t *ra, const uint32_t *rb, uint32_t *out,
+                                         const int32_t *__restrict__ nRef, unsigned long long *tcuts, uint32_t *err,
                                          const uint8_t *__restrict__ sigBits) {
-    const uint32_t l0 = f0[idx], l1 = f1[idx];
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
     DCut a, b;
     if (MODE == REC_TAGGED) {
@@ -360,8 +360,8 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
             __builtin_amdgcn_s_sleep(1);
         }
     } else {
-        unpack_cut(cuts + ((size_t)in0 * NCUT + (lane >> 3)) * 5, a);
-        unpack_cut(cuts + ((size_t)in1 * NCUT + (lane & 7)) * 5, b);
+        unpack_cut(ra + (lane >> 3) * 5, a);
+        unpack_cut(rb + (lane & 7) * 5, b);
     }
     // ---- phase 1: every pair in parallel
     int status = C_SKIP, shrunk = 0;
@@ -452,11 +452,20 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         if (MODE == REC_TAGGED) {
             store_tagged(tcuts + ((size_t)idx * NCUT + lane) * 3, o0, o1, o2, o3, meta);
         } else {
-            uint32_t *o = cuts + ((size_t)idx * NCUT + lane) * 5;
+            uint32_t *o = out + lane * 5;
             o[0] = o0; o[1] = o1; o[2] = o2; o[3] = o3; o[4] = meta;
         }
     }
     return true;
+}
+
+// Cut set of node idx from the records of its fanins in `cuts`, into `cuts`.
+__device__ __forceinline__ void cut_node_global(const int idx, const int lane, const uint32_t *__restrict__ f0,
+                                                const uint32_t *__restrict__ f1, const int32_t *__restrict__ nRef,
+                                                uint32_t *cuts, const uint8_t *__restrict__ sigBits) {
+    const uint32_t l0 = f0[idx], l1 = f1[idx];
+    cut_node<REC_PLAIN>(idx, lane, l0, l1, cuts + (size_t)LID(l0) * NCUT * 5, cuts + (size_t)LID(l1) * NCUT * 5,
+                        cuts + (size_t)idx * NCUT * 5, nRef, nullptr, nullptr, sigBits);
 }
 
 __global__ void __launch_bounds__(CUTL_BLOCK) k_cuts_level(int32_t start, int32_t count,
@@ -466,7 +475,132 @@ __global__ void __launch_bounds__(CUTL_BLOCK) k_cuts_level(int32_t start, int32_
                                                            const uint8_t *__restrict__ sigBits) {
     const int k = blockIdx.x * (CUTL_BLOCK / 64) + (threadIdx.x >> 6);
     if (k >= count) return;    // uniform per wavefront
-    cut_node<REC_PLAIN>(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
+    cut_node_global(start + k, threadIdx.x & 63, f0, f1, nRef, cuts, sigBits);
+}
+
+// Two levels in one launch.  Workgroups 0..n0-1 compute the nodes of the first
+// level (ids s0..s0+n0-1) as k_cuts_level does.  Workgroup n0 + k takes node
+// s1 + k of the second level: wavefront w first recomputes the cut set of
+// fanin w if that fanin lies on the first level (its own fanins are on earlier
+// levels, whose records are complete), into LDS; after the barrier wavefront 0
+// computes the node from those LDS records and the global records of fanins on
+// earlier levels.  Every computation is the deterministic cut_node, so the
+// recomputed sets equal the stored ones, and one launch replaces two dependent
+// ones (a dependent launch costs about as much as a node's whole computation).
+constexpr int CUTP_BLOCK = 128;
+__global__ void __launch_bounds__(CUTP_BLOCK) k_cuts_pair(int32_t s0, int32_t n0, int32_t s1,
+                                                          const uint32_t *__restrict__ f0,
+                                                          const uint32_t *__restrict__ f1,
+                                                          const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                          const uint8_t *__restrict__ sigBits) {
+    __shared__ uint32_t s_rec[2][NCUT * 5];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = (int)blockIdx.x;
+    if (k < n0) {                       // uniform per workgroup
+        if (w == 0) cut_node_global(s0 + k, lane, f0, f1, nRef, cuts, sigBits);
+        return;
+    }
+    const int v = s1 + (k - n0);
+    const uint32_t l0 = f0[v], l1 = f1[v];
+    const uint32_t in0 = LID(l0), in1 = LID(l1);
+    const uint32_t mine = w == 0 ? in0 : in1;
+    if (mine >= (uint32_t)s0) {         // on the first level (fanins of v are below s1)
+        const uint32_t g0 = f0[mine], g1 = f1[mine];
+        cut_node<REC_PLAIN>((int)mine, lane, g0, g1, cuts + (size_t)LID(g0) * NCUT * 5,
+                            cuts + (size_t)LID(g1) * NCUT * 5, s_rec[w], nRef, nullptr, nullptr, sigBits);
+    }
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t *ra = in0 >= (uint32_t)s0 ? s_rec[0] : cuts + (size_t)in0 * NCUT * 5;
+        const uint32_t *rb = in1 >= (uint32_t)s0 ? s_rec[1] : cuts + (size_t)in1 * NCUT * 5;
+        cut_node<REC_PLAIN>(v, lane, l0, l1, ra, rb, cuts + (size_t)v * NCUT * 5, nRef, nullptr, nullptr, sigBits);
+    }
+}
+
+// Three levels in one launch (ids s0..s1-1, s1..s2-1, s2..e-1), one workgroup
+// per node.  A node of the first level is computed as in k_cuts_level, one of
+// the second as in k_cuts_pair.  A node v of the third level recomputes its
+// cone in its workgroup: the bottom nodes (first-level fanins of v and of v's
+// second-level fanins, at most 4, one wavefront each), then the middle nodes
+// (v's second-level fanins, at most 2), then v, with a barrier between the
+// steps and the recomputed sets held in LDS.  Fanins below the launch are read
+// from the global records, which earlier launches completed.
+constexpr int CUT3_BLOCK = 256;
+__device__ __forceinline__ const uint32_t *rec_or_lds(uint32_t x, const uint32_t *cuts, uint32_t (*s_rec)[NCUT * 5],
+                                                      uint32_t b0, uint32_t b1, uint32_t b2, uint32_t m0,
+                                                      uint32_t m1, uint32_t s0, uint32_t s1) {
+    if (x >= s1) return s_rec[x == m0 ? 4 : 5];
+    if (x >= s0) return s_rec[x == b0 ? 0 : x == b1 ? 1 : x == b2 ? 2 : 3];
+    return cuts + (size_t)x * NCUT * 5;
+}
+
+__global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_triple(int32_t s0_, int32_t s1_, int32_t s2_,
+                                                            const uint32_t *__restrict__ f0,
+                                                            const uint32_t *__restrict__ f1,
+                                                            const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                            const uint8_t *__restrict__ sigBits) {
+    __shared__ uint32_t s_rec[6][NCUT * 5];     // bottom nodes 0..3, middle nodes 4..5
+    const uint32_t s0 = (uint32_t)s0_, s1 = (uint32_t)s1_, s2 = (uint32_t)s2_;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t v = s0 + blockIdx.x;
+    if (v < s1) {                               // uniform per workgroup
+        if (w == 0) cut_node_global((int)v, lane, f0, f1, nRef, cuts, sigBits);
+        return;
+    }
+    const uint32_t l0 = f0[v], l1 = f1[v];
+    const uint32_t a = LID(l0), b = LID(l1);
+    const uint32_t NONE = 0xFFFFFFFFu;
+    // middle nodes (second level) and bottom nodes (first level) of the cone
+    uint32_t m0 = NONE, m1 = NONE, ml0 = 0u, ml1 = 0u, ml2 = 0u, ml3 = 0u;    // fanin literals of m0, m1
+    uint32_t b0 = NONE, b1 = NONE, b2 = NONE, b3 = NONE;
+    int nb = 0;
+    // appends x if it is a first-level node not yet listed (selects, so the
+    // list stays in registers)
+    auto add_bot = [&](uint32_t x) {
+        const bool add = x >= s0 && x != b0 && x != b1 && x != b2;
+        b0 = add && nb == 0 ? x : b0;
+        b1 = add && nb == 1 ? x : b1;
+        b2 = add && nb == 2 ? x : b2;
+        b3 = add && nb == 3 ? x : b3;
+        nb += add ? 1 : 0;
+    };
+    if (v >= s2) {
+        if (a >= s1) m0 = a;
+        if (b >= s1) { if (m0 == NONE) m0 = b; else m1 = b; }
+        if (m0 != NONE) { ml0 = f0[m0]; ml1 = f1[m0]; }
+        if (m1 != NONE) { ml2 = f0[m1]; ml3 = f1[m1]; }
+        if (a < s1) add_bot(a);
+        if (b < s1) add_bot(b);
+        if (m0 != NONE) { add_bot(LID(ml0)); add_bot(LID(ml1)); }
+        if (m1 != NONE) { add_bot(LID(ml2)); add_bot(LID(ml3)); }
+    } else {
+        add_bot(a);
+        add_bot(b);
+    }
+    // step 1: bottom nodes from the global records
+    const uint32_t mine = w == 0 ? b0 : w == 1 ? b1 : w == 2 ? b2 : b3;
+    if (w < nb) {
+        const uint32_t g0 = f0[mine], g1 = f1[mine];
+        cut_node<REC_PLAIN>((int)mine, lane, g0, g1, cuts + (size_t)LID(g0) * NCUT * 5,
+                            cuts + (size_t)LID(g1) * NCUT * 5, s_rec[w], nRef, nullptr, nullptr, sigBits);
+    }
+    __syncthreads();
+    // step 2: middle nodes
+    if (v >= s2) {
+        const uint32_t mm = w == 0 ? m0 : m1;
+        if (w < 2 && mm != NONE) {
+            const uint32_t g0 = w == 0 ? ml0 : ml2, g1 = w == 0 ? ml1 : ml3;
+            cut_node<REC_PLAIN>((int)mm, lane, g0, g1, rec_or_lds(LID(g0), cuts, s_rec, b0, b1, b2, m0, m1, s0, s1),
+                                rec_or_lds(LID(g1), cuts, s_rec, b0, b1, b2, m0, m1, s0, s1), s_rec[4 + w], nRef,
+                                nullptr, nullptr, sigBits);
+        }
+        __syncthreads();
+    }
+    // step 3: the node itself
+    if (w == 0)
+        cut_node<REC_PLAIN>((int)v, lane, l0, l1, rec_or_lds(a, cuts, s_rec, b0, b1, b2, m0, m1, s0, s1),
+                            rec_or_lds(b, cuts, s_rec, b0, b1, b2, m0, m1, s0, s1), cuts + (size_t)v * NCUT * 5, nRef,
+                            nullptr, nullptr, sigBits);
 }
 
 // All levels in one cooperative launch: the wavefronts of the grid stride over
@@ -482,7 +616,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_levels(int32_t l0, int32_t 
     for (int l = l0; l < l1; l++) {
         const int s = starts[l], e = starts[l + 1];
         for (int v = s + gw; v < e; v += nw)
-            cut_node<REC_PLAIN>(v, threadIdx.x & 63, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
+            cut_node_global(v, threadIdx.x & 63, f0, f1, nRef, cuts, sigBits);
         grid.sync();
     }
 }
@@ -542,7 +676,9 @@ __global__ void __launch_bounds__(XCD_WG) k_cuts_xcd(int32_t l0, int32_t l1, con
     const uint32_t wg = blockIdx.x / XCD_COUNT;
     if (wg >= nwg) return;
     const int lane = threadIdx.x & 63;
-    const int gw = (int)wg * (XCD_WG / 64) + (int)(threadIdx.x >> 6);
+    // a level's nodes go round-robin over the workgroups (one per CU) first,
+    // so a level of n nodes puts about n / nwg wavefronts on each CU
+    const int gw = (int)(threadIdx.x >> 6) * (int)nwg + (int)wg;
     const int nw = (int)nwg * (XCD_WG / 64);
     uint32_t epoch = 0;
     if (threadIdx.x == 0) __hip_atomic_fetch_or(&ctl[1], 1u << xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -554,7 +690,7 @@ __global__ void __launch_bounds__(XCD_WG) k_cuts_xcd(int32_t l0, int32_t l1, con
     for (int l = l0; l < l1; l++) {
         const int s = starts[l], e = starts[l + 1];
         for (int v = s + gw; v < e; v += nw)
-            cut_node<REC_PLAIN>(v, lane, f0, f1, nRef, cuts, nullptr, nullptr, sigBits);
+            cut_node_global(v, lane, f0, f1, nRef, cuts, sigBits);
         if (!xcd_barrier(ctl, epoch, nwg, &s_stop)) return;
     }
 }
@@ -579,7 +715,7 @@ __global__ void __launch_bounds__(CUTW_BLOCK) k_cuts_dataflow(int32_t first, int
         t = (uint32_t)__shfl((int)t, 0);
         const int v = first + (int)t;
         if (v >= last) break;
-        if (!cut_node<REC_TAGGED>(v, lane, f0, f1, nRef, nullptr, tcuts, err, sigBits)) break;
+        if (!cut_node<REC_TAGGED>(v, lane, f0[v], f1[v], nullptr, nullptr, nullptr, nRef, tcuts, err, sigBits)) break;
     }
 }
 
@@ -1416,13 +1552,18 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // and its last the stop event, so the elapsed time over the launches is
         // the per-launch kernel time a profiler reports, with the events' own
         // cost spread over the window
+        // default: three levels per launch (k_cuts_triple); GLS_RW_PAIR_CUTS:
+        // two (k_cuts_pair); GLS_RW_LEVEL_CUTS: one (k_cuts_level)
+        const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : 3;
         int win = 0;
-        for (int l = 1; l < L; l++) {
-            int32_t s = levelStart[l], e = levelStart[l + 1];
+        for (int l = 1; l < L; l += step) {
+            const int32_t s = levelStart[l], m = levelStart[std::min(l + 1, L)];
+            const int32_t m2 = levelStart[std::min(l + 2, L)];
+            const int32_t e = levelStart[std::min(l + step, L)];
             if (e <= s) continue;
-            const dim3 grid(blocks(e - s, CUTL_BLOCK / 64));
+            const dim3 grid(step > 1 ? (unsigned)(e - s) : blocks(e - s, CUTL_BLOCK / 64));
             hipEvent_t evStart = nullptr, evStop = nullptr;
-            if (kSampleLaunches && win == 0 && l % KSAMPLE_EVERY == 0) {
+            if (kSampleLaunches && win == 0 && (l - 1) % KSAMPLE_EVERY == 0) {
                 const size_t k = 2 * (size_t)ctx->ksamp_n;
                 while (ctx->kev.size() < k + 2) {
                     hipEvent_t ev = nullptr;
@@ -1439,9 +1580,22 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                 ctx->ksamp_units.back() += e - s;
                 ctx->ksamp_launches.back() += 1;
                 if (--win == 0) evStop = ctx->kev[2 * (size_t)ctx->ksamp_n - 1];
-                hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTL_BLOCK), 0, st, evStart, evStop, 0, s, e - s,
-                                      (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
-                                      d_cuts, sigBits);
+                if (step == 3)
+                    hipExtLaunchKernelGGL(k_cuts_triple, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0, s, m, m2,
+                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
+                                          d_cuts, sigBits);
+                else if (step == 2)
+                    hipExtLaunchKernelGGL(k_cuts_pair, grid, dim3(CUTP_BLOCK), 0, st, evStart, evStop, 0, s, m - s, m,
+                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
+                                          d_cuts, sigBits);
+                else
+                    hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTL_BLOCK), 0, st, evStart, evStop, 0, s, e - s,
+                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
+                                          d_cuts, sigBits);
+            } else if (step == 3) {
+                k_cuts_triple<<<grid, CUT3_BLOCK, 0, st>>>(s, m, m2, g.f0, g.f1, d_nref, d_cuts, sigBits);
+            } else if (step == 2) {
+                k_cuts_pair<<<grid, CUTP_BLOCK, 0, st>>>(s, m - s, m, g.f0, g.f1, d_nref, d_cuts, sigBits);
             } else {
                 k_cuts_level<<<grid, CUTL_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
@@ -1530,7 +1684,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             GLS_TRY(sig_bits_for_tags(ctx, n, tagR, sigBits));
         }
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
-                                   flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS),
+                                   flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS |
+                                            GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS),
                                    sigBits));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};

@@ -106,7 +106,10 @@ int gls_strash(gls_aig *aig, int32_t flags);
  * rwz).  Replaces AIGMan::rewrite (src/aig_manager.cu:632-672) and
  * rw::CPUSolver::Rewrite (src/algorithms/rewrite.cu:811-938).  The AIG stays on
  * the device; the result is strashed.  flags select how cut enumeration is
- * scheduled (the results are identical): 0 = one kernel launch per level,
+ * scheduled (the results are identical): 0 = one kernel launch per three
+ * levels (a node recomputes the cut sets of its fanin cone inside the launch
+ * in its own workgroup), GLS_RW_PAIR_CUTS = the same with two levels per
+ * launch, GLS_RW_LEVEL_CUTS = one kernel launch per level,
  * GLS_RW_COOP_CUTS = one cooperative launch with a grid barrier per level,
  * GLS_RW_DATAFLOW_CUTS = persistent wavefronts that wait only for their fanins,
  * GLS_RW_XCD_CUTS = one persistent kernel on the CUs of one XCD with a barrier
@@ -114,6 +117,8 @@ int gls_strash(gls_aig *aig, int32_t flags);
 #define GLS_RW_COOP_CUTS 2
 #define GLS_RW_DATAFLOW_CUTS 4
 #define GLS_RW_XCD_CUTS 8
+#define GLS_RW_LEVEL_CUTS 16
+#define GLS_RW_PAIR_CUTS 32
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing

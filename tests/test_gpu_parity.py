@@ -94,18 +94,21 @@ def test_rewrite_parity(gpu_ctx, name, spec):
 
 @pytest.mark.parametrize("name,spec", CASES[1:], ids=[c[0] for c in CASES[1:]])
 def test_cut_launch_modes_agree(gpu_ctx, name, spec):
-    """One launch per level, one cooperative launch, the dataflow and the
-    XCD-local schedules agree."""
+    """Three and two levels per launch, one launch per level, one
+    cooperative launch, the dataflow and the XCD-local schedules agree."""
     import culs_amd
     log("cut launch modes: %s" % name)
     a = make(gpu_ctx, spec)
     a.strash()
-    b, c, d = a.copy(), a.copy(), a.copy()
+    b, c, d, e, f = a.copy(), a.copy(), a.copy(), a.copy(), a.copy()
     a.rewrite(cut_mode=0)
     b.rewrite(cut_mode=culs_amd.GLS_RW_DATAFLOW_CUTS)
     c.rewrite(cut_mode=culs_amd.GLS_RW_COOP_CUTS)
     d.rewrite(cut_mode=culs_amd.GLS_RW_XCD_CUTS)
-    assert same(a.arrays(), b.arrays()) and same(a.arrays(), c.arrays()) and same(a.arrays(), d.arrays())
+    e.rewrite(cut_mode=culs_amd.GLS_RW_LEVEL_CUTS)
+    f.rewrite(cut_mode=culs_amd.GLS_RW_PAIR_CUTS)
+    for x in (b, c, d, e, f):
+        assert same(a.arrays(), x.arrays())
 
 
 @pytest.mark.parametrize("name,spec", CASES[:4], ids=[c[0] for c in CASES[:4]])
