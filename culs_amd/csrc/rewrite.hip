@@ -137,8 +137,10 @@ __device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut 
         k += takeB;
     }
     if (i < a.nLeaves || k < b.nLeaves) return 0;
+    // positions past the count repeat the first leaf (the containment tests of
+    // the slot replay rely on it; stored records zero them)
 #pragma unroll
-    for (int p = 0; p < 4; p++) c.leaves[p] = o[p];
+    for (int p = 0; p < 4; p++) c.leaves[p] = p < n ? o[p] : o[0];
     c.nLeaves = n;
     return 1;
 }
@@ -185,7 +187,7 @@ __device__ __forceinline__ int minimize_support(DCut &c) {
     t = tt_shrink(t, phase);
     c.tt = t & 0xFFFF;
     // compaction: kept leaf i moves to slot popc(phase below i); the slots past
-    // the new count keep their old contents, as an in-place compaction would
+    // the new count repeat the first leaf, as after the merge
     uint32_t o[4];
 #pragma unroll
     for (int p = 0; p < 4; p++) o[p] = c.leaves[p];
@@ -197,7 +199,7 @@ __device__ __forceinline__ int minimize_support(DCut &c) {
             if (p == pos && ((phase >> i) & 1)) o[p] = c.leaves[i];
     }
 #pragma unroll
-    for (int p = 0; p < 4; p++) c.leaves[p] = o[p];
+    for (int p = 0; p < 4; p++) c.leaves[p] = p < nL ? o[p] : o[0];
     c.nLeaves = nL;
     return 1;
 }
@@ -238,10 +240,6 @@ enum : int { C_SKIP = 0, C_FAIL = 1, C_OK = 2 };
 
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-__device__ __forceinline__ bool leaf_in(uint32_t x, uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, int n) {
-    return (n > 0 && x == s0) || (n > 1 && x == s1) || (n > 2 && x == s2) || (n > 3 && x == s3);
-}
-
 // Minimum of v over lanes 0..7, valid in lane 0: two quad_perm steps (xor 1,
 // xor 2) and row rotations by 4 and 12, so lane 0 also sees lanes 4..7 in
 // either rotation direction.  DPP moves, no LDS round trip.
@@ -276,16 +274,19 @@ __device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t 
     // prefilter, rewrite.cu:181, 185): when no slot passes it, nothing changes
     const bool maybe = mine && (bigger ? (s.sig & nsig) == nsig : (s.sig & nsig) == s.sig);
     if (__ballot(maybe) == 0) return false;
-    bool d1 = false, d2 = false;
-    if (maybe) {
-        if (bigger) {
-            d1 = (nn < 1 || leaf_in(n0, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 2 || leaf_in(n1, s.l0, s.l1, s.l2, s.l3, s.n)) &&
-                 (nn < 3 || leaf_in(n2, s.l0, s.l1, s.l2, s.l3, s.n)) && (nn < 4 || leaf_in(n3, s.l0, s.l1, s.l2, s.l3, s.n));
-        } else {
-            d2 = (s.n < 1 || leaf_in(s.l0, n0, n1, n2, n3, nn)) && (s.n < 2 || leaf_in(s.l1, n0, n1, n2, n3, nn)) &&
-                 (s.n < 3 || leaf_in(s.l2, n0, n1, n2, n3, nn)) && (s.n < 4 || leaf_in(s.l3, n0, n1, n2, n3, nn));
-        }
-    }
+    // exact containment both ways from one 4x4 equality matrix: leaf lists are
+    // padded with their first leaf past their count, so a padded entry repeats
+    // a real one and no count enters the test (bitwise, without branches)
+    const bool e00 = n0 == s.l0, e01 = n0 == s.l1, e02 = n0 == s.l2, e03 = n0 == s.l3;
+    const bool e10 = n1 == s.l0, e11 = n1 == s.l1, e12 = n1 == s.l2, e13 = n1 == s.l3;
+    const bool e20 = n2 == s.l0, e21 = n2 == s.l1, e22 = n2 == s.l2, e23 = n2 == s.l3;
+    const bool e30 = n3 == s.l0, e31 = n3 == s.l1, e32 = n3 == s.l2, e33 = n3 == s.l3;
+    const bool nInS = (nn == 0) | ((e00 | e01 | e02 | e03) & (e10 | e11 | e12 | e13) & (e20 | e21 | e22 | e23) &
+                                   (e30 | e31 | e32 | e33));
+    const bool sInN = (s.n == 0) | ((nn != 0) & (e00 | e10 | e20 | e30) & (e01 | e11 | e21 | e31) &
+                                    (e02 | e12 | e22 | e32) & (e03 | e13 | e23 | e33));
+    const bool d1 = maybe & bigger & nInS;
+    const bool d2 = maybe & !bigger & sInN;
     const uint64_t m2 = __ballot(d2);
     const int k = m2 ? __ffsll((unsigned long long)m2) - 1 : NCUT;
     if (d1 && lane < k) s.used = 0;
@@ -395,7 +396,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     // ---- phase 2: replay the insertions in (i, j) order
     Slot s;
     s.l0 = lane == 0 ? (uint32_t)idx : 0u;
-    s.l1 = s.l2 = s.l3 = 0;
+    s.l1 = s.l2 = s.l3 = s.l0;          // padded with the first leaf
     s.n = lane == 0 ? 1 : 0;
     s.tt = lane == 0 ? 0xAAAAu : 0u;
     s.value = lane == 0 ? 1001u : 0u;
