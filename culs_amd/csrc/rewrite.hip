@@ -251,11 +251,14 @@ __device__ __forceinline__ uint32_t min8_lane0(uint32_t v) {
     return v;
 }
 
+// One slot of the cut set under construction, held by lane = slot index; which
+// slots are in use is the wave-uniform bit mask `used` of the replay.
 struct Slot {
     uint32_t l0, l1, l2, l3, tt, value;
-    int n, used;
+    int n;
     uint32_t sig;       // bit (leaf & 31) of every leaf: a prefilter for containment
 };
+
 
 __device__ __forceinline__ uint32_t sig4(uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3, int nn) {
     return (nn > 0 ? 1u << (n0 & 31) : 0u) | (nn > 1 ? 1u << (n1 & 31) : 0u) | (nn > 2 ? 1u << (n2 & 31) : 0u) |
@@ -267,12 +270,12 @@ __device__ __forceinline__ uint32_t sig4(uint32_t n0, uint32_t n1, uint32_t n2, 
 // is dropped; the first slot of no more leaves that N contains drops N and ends
 // the scan.  Returns true if N was dropped.
 __device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t n0, uint32_t n1, uint32_t n2,
-                                            uint32_t n3, int nn, uint32_t nsig) {
-    const bool mine = lane < NCUT && lane != id && s.used;
+                                            uint32_t n3, int nn, uint32_t nsig, uint32_t &used) {
+    const bool mine = lane < NCUT && lane != id && ((used >> (lane & 7)) & 1u);
     const bool bigger = s.n > nn;
     // containment implies containment of the leaf signatures (CutFilter's own
     // prefilter, rewrite.cu:181, 185): when no slot passes it, nothing changes
-    const bool maybe = mine && (bigger ? (s.sig & nsig) == nsig : (s.sig & nsig) == s.sig);
+    const bool maybe = mine & ((s.sig & nsig) == (bigger ? nsig : s.sig));
     if (__ballot(maybe) == 0) return false;
     // exact containment both ways from one 4x4 equality matrix: leaf lists are
     // padded with their first leaf past their count, so a padded entry repeats
@@ -289,9 +292,9 @@ __device__ __forceinline__ bool filter_wave(Slot &s, int lane, int id, uint32_t 
     const bool d2 = maybe & !bigger & sInN;
     const uint64_t m2 = __ballot(d2);
     const int k = m2 ? __ffsll((unsigned long long)m2) - 1 : NCUT;
-    if (d1 && lane < k) s.used = 0;
+    used &= ~(uint32_t)(__ballot(d1 && lane < k) & 0xFFull);
     if (k < NCUT) {
-        if (lane == id) s.used = 0;
+        used &= ~(1u << id);
         return true;
     }
     return false;
@@ -400,8 +403,8 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     s.n = lane == 0 ? 1 : 0;
     s.tt = lane == 0 ? 0xAAAAu : 0u;
     s.value = lane == 0 ? 1001u : 0u;
-    s.used = lane == 0 ? 1 : 0;
     s.sig = lane == 0 ? 1u << ((uint32_t)idx & 31) : 0u;
+    uint32_t used = 1u;                 // slots in use (bit = slot), wave-uniform
     uint64_t todo = __ballot(status != C_SKIP);
     while (todo) {
         const int l = __ffsll((unsigned long long)todo) - 1;
@@ -409,10 +412,9 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         const uint32_t inf = (uint32_t)rl((int)info, l);
         const int st = (int)(inf & 3u);
         // FindCut (rewrite.cu:84-109)
-        const uint64_t usedMask = __ballot(lane < NCUT && s.used) & 0xFFull;
         int id;
-        if (usedMask != 0xFFull) {
-            id = __ffsll((unsigned long long)(~usedMask & 0xFFull)) - 1;
+        if (used != 0xFFu) {
+            id = __ffs(~used & 0xFFu) - 1;
         } else {
             // categories in order (more than 2 leaves, 2, fewer), then the
             // smallest value, then the first slot: one minimum of that key
@@ -420,33 +422,36 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
             const uint32_t key = lane < NCUT ? (cat << 14 | (s.value & 0x7FF) << 3 | (uint32_t)lane) : 0xFFFFFFFFu;
             id = (int)((uint32_t)rl((int)min8_lane0(key), 0) & 7u);
         }
-        if (lane == id) s.used = 0;
+        used &= ~(1u << id);
         if (st == C_FAIL) continue;
+        // the slot takes the merged cut (selects on one lane compare)
         const int nn = (int)((inf >> 3) & 7u);
         const uint32_t n0 = (uint32_t)rl((int)m.leaves[0], l), n1 = (uint32_t)rl((int)m.leaves[1], l);
         const uint32_t n2 = (uint32_t)rl((int)m.leaves[2], l), n3 = (uint32_t)rl((int)m.leaves[3], l);
         const uint32_t nsig = (uint32_t)rl((int)msig, l);
-        if (lane == id) { s.l0 = n0; s.l1 = n1; s.l2 = n2; s.l3 = n3; s.n = nn; s.used = 1; s.sig = nsig; }
-        if (filter_wave(s, lane, id, n0, n1, n2, n3, nn, nsig)) continue;
+        const bool at = lane == id;
+        s.l0 = at ? n0 : s.l0; s.l1 = at ? n1 : s.l1; s.l2 = at ? n2 : s.l2; s.l3 = at ? n3 : s.l3;
+        s.n = at ? nn : s.n; s.sig = at ? nsig : s.sig;
+        used |= 1u << id;
+        if (filter_wave(s, lane, id, n0, n1, n2, n3, nn, nsig, used)) continue;
         const uint32_t ttw = (uint32_t)rl((int)tts, l);
-        if (lane == id) s.tt = ttw & 0xFFFFu;
+        s.tt = at ? ttw & 0xFFFFu : s.tt;
         int fn = nn;
         if ((inf >> 2) & 1u) {
             fn = (int)((inf >> 6) & 7u);
             const uint32_t q0 = (uint32_t)rl((int)mm.leaves[0], l), q1 = (uint32_t)rl((int)mm.leaves[1], l);
             const uint32_t q2 = (uint32_t)rl((int)mm.leaves[2], l), q3 = (uint32_t)rl((int)mm.leaves[3], l);
-            const uint32_t qt = ttw >> 16;
             const uint32_t qsig = (uint32_t)rl((int)mmsig, l);
-            if (lane == id) { s.l0 = q0; s.l1 = q1; s.l2 = q2; s.l3 = q3; s.n = fn; s.tt = qt; s.sig = qsig; }
-            filter_wave(s, lane, id, q0, q1, q2, q3, fn, qsig);
+            s.l0 = at ? q0 : s.l0; s.l1 = at ? q1 : s.l1; s.l2 = at ? q2 : s.l2; s.l3 = at ? q3 : s.l3;
+            s.n = at ? fn : s.n; s.tt = at ? ttw >> 16 : s.tt; s.sig = at ? qsig : s.sig;
+            filter_wave(s, lane, id, q0, q1, q2, q3, fn, qsig, used);
         }
-        const uint32_t val = (inf >> 9) & 0x7FFu;
-        if (lane == id) s.value = val;
+        s.value = at ? (inf >> 9) & 0x7FFu : s.value;
         if (fn < 2) break;
     }
     // ---- phase 3: lanes 0..7 store the slots
     if (lane < NCUT) {
-        const bool u = s.used;
+        const bool u = (used >> lane) & 1u;
         const uint32_t o0 = (u && s.n > 0) ? s.l0 : 0u, o1 = (u && s.n > 1) ? s.l1 : 0u;
         const uint32_t o2 = (u && s.n > 2) ? s.l2 : 0u, o3 = (u && s.n > 3) ? s.l3 : 0u;
         const uint32_t meta = u ? ((s.tt & 0xFFFF) | (s.value & 0x7FF) << 16 | (uint32_t)(s.n & 7) << 27 | 1u << 30) : 0u;
