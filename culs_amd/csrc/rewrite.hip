@@ -67,10 +67,20 @@ struct DCut {
 // batched pass sigBits holds each node's bit as it would be in its own AIG
 // (its id there & 31), so that the prefilter of rewrite.cu:283 sees the same
 // signatures as when that AIG is rewritten alone.
+// Without branches per leaf: the sigBits test is uniform, and in the batched
+// case the four byte loads are issued at once (absent leaves read entry 0).
 __device__ __forceinline__ uint32_t cut_sign_of(const DCut &c, const uint8_t *__restrict__ sigBits) {
+    uint32_t b[4];
+    if (sigBits) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) b[i] = sigBits[i < c.nLeaves ? c.leaves[i] : 0u];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) b[i] = c.leaves[i] & 31;
+    }
     uint32_t s = 0;
-    for (int i = 0; i < 4; i++)
-        if (i < c.nLeaves) s |= 1u << (sigBits ? sigBits[c.leaves[i]] : (c.leaves[i] & 31));
+#pragma unroll
+    for (int i = 0; i < 4; i++) s |= i < c.nLeaves ? 1u << b[i] : 0u;
     return s;
 }
 
