@@ -1442,6 +1442,9 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
 // (GLS_KERNEL_SAMPLE=0 in the environment turns this off, e.g. under a
 // profiler that collects counters per dispatch)
 static constexpr int KSAMPLE_EVERY = 64;
+// widest level (nodes) for which cut enumeration recomputes fanin cones to
+// cover several levels per launch
+static constexpr int32_t CUT_WIDE = 2048;
 static constexpr int KSAMPLE_WIN = 8;
 static const bool kSampleLaunches = [] {
     const char *e = getenv("GLS_KERNEL_SAMPLE");
@@ -1563,23 +1566,40 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         pool_put(ctx, d_starts);
         if (e != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cooperative cut launch: %s", hipGetErrorString(e));
     } else {
-        // sampled timing: a window of KSAMPLE_WIN consecutive launches starts
-        // every KSAMPLE_EVERY levels; its first dispatch carries the start event
-        // and its last the stop event, so the elapsed time over the launches is
-        // the per-launch kernel time a profiler reports, with the events' own
-        // cost spread over the window
         // default: three levels per launch (k_cuts_triple); GLS_RW_PAIR_CUTS:
-        // two (k_cuts_pair); GLS_RW_LEVEL_CUTS: one (k_cuts_level)
+        // two (k_cuts_pair); GLS_RW_LEVEL_CUTS: one (k_cuts_level).  A group
+        // with a level wider than CUT_WIDE nodes fills the GPU by itself: the
+        // recomputed cones would only add work there, so its levels get one
+        // launch each.
         const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : 3;
-        int win = 0;
+        struct CutLaunch { int kind; int32_t s, m, m2, e; };
+        std::vector<CutLaunch> plan;
+        plan.reserve((size_t)L / step + 1);
         for (int l = 1; l < L; l += step) {
             const int32_t s = levelStart[l], m = levelStart[std::min(l + 1, L)];
             const int32_t m2 = levelStart[std::min(l + 2, L)];
             const int32_t e = levelStart[std::min(l + step, L)];
             if (e <= s) continue;
-            const dim3 grid(step > 1 ? (unsigned)(e - s) : blocks(e - s, CUTL_BLOCK / 64));
+            if (step > 1 && std::max(std::max(m - s, m2 - m), e - m2) > CUT_WIDE) {
+                for (int j = l; j < std::min(l + step, L); j++)
+                    if (levelStart[j + 1] > levelStart[j])
+                        plan.push_back({1, levelStart[j], levelStart[j + 1], levelStart[j + 1], levelStart[j + 1]});
+            } else {
+                plan.push_back({step, s, m, m2, e});
+            }
+        }
+        // sampled timing of the schedule's own kernel: a window of up to
+        // KSAMPLE_WIN consecutive launches of it starts every KSAMPLE_EVERY
+        // launches; its first dispatch carries the start event and its last the
+        // stop event, so the elapsed time over the launches is the per-launch
+        // kernel time a profiler reports, with the events' own cost spread over
+        // the window
+        int win = 0;
+        for (size_t i = 0; i < plan.size(); i++) {
+            const CutLaunch &c = plan[i];
+            const dim3 grid(c.kind > 1 ? (unsigned)(c.e - c.s) : blocks(c.e - c.s, CUTL_BLOCK / 64));
             hipEvent_t evStart = nullptr, evStop = nullptr;
-            if (kSampleLaunches && win == 0 && (l - 1) % KSAMPLE_EVERY == 0) {
+            if (kSampleLaunches && win == 0 && c.kind == step && i % KSAMPLE_EVERY == 0) {
                 const size_t k = 2 * (size_t)ctx->ksamp_n;
                 while (ctx->kev.size() < k + 2) {
                     hipEvent_t ev = nullptr;
@@ -1593,31 +1613,33 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                 win = KSAMPLE_WIN;
             }
             if (win > 0) {
-                ctx->ksamp_units.back() += e - s;
+                ctx->ksamp_units.back() += c.e - c.s;
                 ctx->ksamp_launches.back() += 1;
-                if (--win == 0) evStop = ctx->kev[2 * (size_t)ctx->ksamp_n - 1];
-                if (step == 3)
-                    hipExtLaunchKernelGGL(k_cuts_triple, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0, s, m, m2,
-                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
-                                          d_cuts, sigBits);
-                else if (step == 2)
-                    hipExtLaunchKernelGGL(k_cuts_pair, grid, dim3(CUTP_BLOCK), 0, st, evStart, evStop, 0, s, m - s, m,
-                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
-                                          d_cuts, sigBits);
+                // the window ends at its last launch or before a launch of another kernel
+                if (--win == 0 || i + 1 == plan.size() || plan[i + 1].kind != step) {
+                    win = 0;
+                    evStop = ctx->kev[2 * (size_t)ctx->ksamp_n - 1];
+                }
+                if (c.kind == 3)
+                    hipExtLaunchKernelGGL(k_cuts_triple, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0, c.s, c.m,
+                                          c.m2, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
+                                          (const int32_t *)d_nref, d_cuts, sigBits);
+                else if (c.kind == 2)
+                    hipExtLaunchKernelGGL(k_cuts_pair, grid, dim3(CUTP_BLOCK), 0, st, evStart, evStop, 0, c.s,
+                                          c.m - c.s, c.m, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
+                                          (const int32_t *)d_nref, d_cuts, sigBits);
                 else
-                    hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTL_BLOCK), 0, st, evStart, evStop, 0, s, e - s,
-                                          (const uint32_t *)g.f0, (const uint32_t *)g.f1, (const int32_t *)d_nref,
-                                          d_cuts, sigBits);
-            } else if (step == 3) {
-                k_cuts_triple<<<grid, CUT3_BLOCK, 0, st>>>(s, m, m2, g.f0, g.f1, d_nref, d_cuts, sigBits);
-            } else if (step == 2) {
-                k_cuts_pair<<<grid, CUTP_BLOCK, 0, st>>>(s, m - s, m, g.f0, g.f1, d_nref, d_cuts, sigBits);
+                    hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTL_BLOCK), 0, st, evStart, evStop, 0, c.s,
+                                          c.e - c.s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
+                                          (const int32_t *)d_nref, d_cuts, sigBits);
+            } else if (c.kind == 3) {
+                k_cuts_triple<<<grid, CUT3_BLOCK, 0, st>>>(c.s, c.m, c.m2, g.f0, g.f1, d_nref, d_cuts, sigBits);
+            } else if (c.kind == 2) {
+                k_cuts_pair<<<grid, CUTP_BLOCK, 0, st>>>(c.s, c.m - c.s, c.m, g.f0, g.f1, d_nref, d_cuts, sigBits);
             } else {
-                k_cuts_level<<<grid, CUTL_BLOCK, 0, st>>>(s, e - s, g.f0, g.f1, d_nref, d_cuts, sigBits);
+                k_cuts_level<<<grid, CUTL_BLOCK, 0, st>>>(c.s, c.e - c.s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
         }
-        // a window cut short by the last level ends after its last launch
-        if (win > 0) GLS_HIP(ctx, hipEventRecord(ctx->kev[2 * (size_t)ctx->ksamp_n - 1], st));
     }
     GLS_HIP(ctx, hipGetLastError());
     return GLS_OK;
