@@ -43,38 +43,6 @@ __device__ __forceinline__ int64_t ht_insert_min(unsigned long long *keys, unsig
     return -1;
 }
 
-// Insert-if-absent for a table whose keys are unique (no value race).
-__device__ __forceinline__ void ht_insert_unique(unsigned long long *keys, uint32_t *vals,
-                                                 uint64_t mask, uint64_t key, uint32_t val,
-                                                 uint32_t *err) {
-    uint64_t s = mix64(key) & mask;
-    for (uint64_t probe = 0; probe <= mask; ++probe) {
-        unsigned long long prev = atomicCAS(&keys[s], KEY_EMPTY, (unsigned long long)key);
-        if (prev == KEY_EMPTY || prev == key) {
-            vals[s] = val;
-            return;
-        }
-        s = (s + 1) & mask;
-    }
-    atomicOr(err, 1u);
-}
-
-template <class V>
-__device__ __forceinline__ bool ht_find(const unsigned long long *keys, const V *vals, uint64_t mask,
-                                        uint64_t key, V *out) {
-    uint64_t s = mix64(key) & mask;
-    for (uint64_t probe = 0; probe <= mask; ++probe) {
-        unsigned long long k = keys[s];
-        if (k == key) {
-            *out = vals[s];
-            return true;
-        }
-        if (k == KEY_EMPTY) return false;
-        s = (s + 1) & mask;
-    }
-    return false;
-}
-
 // Frontier controller of a level-synchronous sweep.  order[] holds the
 // frontiers back to back: F_{k-1} = [prevStart, +prevSize), F_k = [curStart,
 // +curSize), F_{k+1} is appended after F_k.  The last block of each round

@@ -772,12 +772,26 @@ __global__ void k_nref(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
     for (int64_t i = tid; i < nPOs; i += stride) atomicAdd(&nRef[LID(pos[i])], 1);
 }
 
-__global__ void k_table_build(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
-                              const uint32_t *__restrict__ f1, unsigned long long *keys, uint32_t *vals,
-                              uint64_t mask, uint32_t *err) {
+// Structural table of the graph, indexed by fanin: bucket x holds the ANDs
+// whose second fanin literal (the one of larger id) is on node x, as entries
+// {first fanin literal, node | second literal's complement << 31}.  Buckets
+// follow node ids, so the lookups of nearby nodes touch nearby memory (a hash
+// table of the same keys spreads them over the whole table, which for the
+// batched arena is far larger than the caches).
+__global__ void k_table_count(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f1, uint32_t *cnt) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = nPIs + 1 + tid; v < n; v += stride)
-        ht_insert_unique(keys, vals, mask, and_key(f0[v], f1[v]), (uint32_t)v, err);
+    for (int64_t v = nPIs + 1 + tid; v < n; v += stride) atomicAdd(&cnt[LID(f1[v])], 1u);
+}
+
+__global__ void k_table_fill(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0,
+                             const uint32_t *__restrict__ f1, const uint32_t *__restrict__ off, uint32_t *cur,
+                             uint2 *ent) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
+        const uint32_t b = f1[v];
+        const uint32_t k = off[LID(b)] + atomicAdd(&cur[LID(b)], 1u);
+        ent[k] = make_uint2(f0[v], (uint32_t)v | (b & 1u) << 31);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -787,9 +801,8 @@ struct GraphView {
     const uint32_t *f0, *f1;
     const int32_t *nRef;
     const uint32_t *level;
-    const unsigned long long *htK;
-    const uint32_t *htV;
-    uint64_t mask;
+    const uint32_t *tOff;       // structural table: bucket offsets (n + 1)
+    const uint2 *tEnt;          // and entries (see k_table_fill)
 };
 
 // per-thread column of an LDS array laid out [j][lane]
@@ -881,12 +894,18 @@ __device__ __forceinline__ int mffc_deleted(const Mffc &t, int x) {
     return r;
 }
 
+// The AND with fanins (in0, c0), (in1, c1) if the graph has one (the smallest
+// such id), else -1: a scan of the bucket of the fanin of larger id.
 __device__ __forceinline__ int table_lookup(const GraphView &g, int in0, int in1, int c0, int c1) {
     if (in0 > in1) { int x = in0; in0 = in1; in1 = x; x = c0; c0 = c1; c1 = x; }
-    uint32_t v;
-    if (ht_find(g.htK, g.htV, g.mask, and_key(2u * (uint32_t)in0 + (uint32_t)c0, 2u * (uint32_t)in1 + (uint32_t)c1), &v))
-        return (int)v;
-    return -1;
+    const uint32_t lo = 2u * (uint32_t)in0 + (uint32_t)c0, hc = (uint32_t)c1 << 31;
+    const uint32_t e = g.tOff[in1 + 1];
+    uint32_t best = 0xFFFFFFFFu;
+    for (uint32_t k = g.tOff[in1]; k < e; k++) {
+        const uint2 x = g.tEnt[k];
+        if (x.x == lo && (x.y & 0x80000000u) == hc) best = min(best, x.y & 0x7FFFFFFFu);
+    }
+    return best == 0xFFFFFFFFu ? -1 : (int)best;
 }
 
 // library matching of one cut (rewrite.cu:391-418); returns the class
@@ -1645,19 +1664,25 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     return GLS_OK;
 }
 
-static int build_table(gls_ctx *ctx, Tmp &t, const DevGraph &g, GraphView &gv, uint32_t *d_err) {
+static int build_table(gls_ctx *ctx, Tmp &t, const DevGraph &g, GraphView &gv) {
     hipStream_t st = ctx->stream;
-    uint64_t cap = 1024;
-    while (cap < 2ull * (uint64_t)g.nObjs) cap <<= 1;
-    unsigned long long *keys = t.get<unsigned long long>(cap);
-    uint32_t *vals = t.get<uint32_t>(cap);
-    if (!keys || !vals) return gls_fail(ctx, GLS_ERR_NOMEM, "table allocation");
-    GLS_HIP(ctx, hipMemsetAsync(keys, 0xFF, sizeof(unsigned long long) * cap, st));
-    k_table_build<<<grid_for(g.nObjs), kBlock, 0, st>>>(g.nObjs, g.nPIs, g.f0, g.f1, keys, vals, cap - 1, d_err);
+    const int32_t n = g.nObjs;
+    const size_t nAnd = n > g.nPIs + 1 ? (size_t)(n - g.nPIs - 1) : 0;
+    uint32_t *cnt = t.get<uint32_t>((size_t)n + 1), *off = t.get<uint32_t>((size_t)n + 1);
+    uint2 *ent = t.get<uint2>(std::max<size_t>(nAnd, 1));
+    if (!cnt || !off || !ent) return gls_fail(ctx, GLS_ERR_NOMEM, "table allocation");
+    GLS_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(uint32_t) * ((size_t)n + 1), st));
+    k_table_count<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f1, cnt);
+    size_t tb = 0;
+    GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, cnt, off, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
+    void *tmp = t.get<uint8_t>(std::max<size_t>(tb, 1));
+    if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "table scan storage");
+    GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, cnt, off, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
+    GLS_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(uint32_t) * ((size_t)n + 1), st));
+    k_table_fill<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f0, g.f1, off, cnt, ent);
     GLS_HIP(ctx, hipGetLastError());
-    gv.htK = keys;
-    gv.htV = vals;
-    gv.mask = cap - 1;
+    gv.tOff = off;
+    gv.tEnt = ent;
     return GLS_OK;
 }
 
@@ -1666,11 +1691,8 @@ int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const 
     GLS_TRY(ctx_ensure_library(ctx));
     hipStream_t st = ctx->stream;
     Tmp t{ctx};
-    uint32_t *d_err = t.get<uint32_t>(1);
-    if (!d_err) return gls_fail(ctx, GLS_ERR_NOMEM, "eval");
-    GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
-    GraphView gv{g.f0, g.f1, d_nref, g.level, nullptr, nullptr, 0};
-    GLS_TRY(build_table(ctx, t, g, gv, d_err));
+    GraphView gv{g.f0, g.f1, d_nref, g.level, nullptr, nullptr};
+    GLS_TRY(build_table(ctx, t, g, gv));
     GLS_HIP(ctx, hipMemsetAsync(d_best, 0xFF, sizeof(int4) * 1, st));
     int32_t nAnd = g.nObjs - g.nPIs - 1;
     // PIs and the constant have no candidate
@@ -1726,8 +1748,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
                                             GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS),
                                    sigBits));
         tm.mark(); // 2
-        GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr, 0};
-        GLS_TRY(build_table(ctx, t, g, gv, d_err));
+        GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr};
+        GLS_TRY(build_table(ctx, t, g, gv));
         tm.mark(); // 3
         std::vector<int4> none(nPIs + 1, make_int4(-1, 0, 0, 0));
         GLS_HIP(ctx, hipMemcpyAsync(best, none.data(), sizeof(int4) * none.size(), hipMemcpyHostToDevice, st));
@@ -1753,8 +1775,6 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         GLS_HIP(ctx, hipMemcpyAsync(&nCand, d_cnt, 4, hipMemcpyDeviceToHost, st));
         GLS_HIP(ctx, hipStreamSynchronize(st));
         uint32_t herr = 0;
-        GLS_TRY(stream_copy(ctx, &herr, d_err, 4, hipMemcpyDeviceToHost));
-        if (herr) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: table overflow"); }
         uint32_t nSel = 0, rounds = 0, totalNew = 0;
         unsigned long long estGain = 0;
         k_iota2<<<grid_for(n), kBlock, 0, st>>>(n, repl);
