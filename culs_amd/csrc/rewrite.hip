@@ -559,10 +559,15 @@ __global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_triple(int32_t s0_, int32_t
     const uint32_t s0 = (uint32_t)s0_, s1 = (uint32_t)s1_, s2 = (uint32_t)s2_;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t v = s0 + blockIdx.x;
+    // issue priority by chain length: a third-level node's workgroup runs three
+    // dependent steps and bounds the launch, a first-level node's runs one
     if (v < s1) {                               // uniform per workgroup
+        __builtin_amdgcn_s_setprio(0);
         if (w == 0) cut_node_global((int)v, lane, f0, f1, nRef, cuts, sigBits);
         return;
     }
+    if (v >= s2) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(1);
     const uint32_t l0 = f0[v], l1 = f1[v];
     const uint32_t a = LID(l0), b = LID(l1);
     const uint32_t NONE = 0xFFFFFFFFu;
