@@ -84,13 +84,6 @@ __device__ __forceinline__ uint32_t cut_sign_of(const DCut &c, const uint8_t *__
     return s;
 }
 
-__device__ __forceinline__ uint32_t cut_sign(const DCut &c) {
-    uint32_t s = 0;
-    for (int i = 0; i < 4; i++)
-        if (i < c.nLeaves) s |= 1u << (c.leaves[i] & 31);
-    return s;
-}
-
 __device__ __forceinline__ uint32_t cut_value(const DCut &c, const int32_t *nRef) {
     // all four loads in flight at once; absent leaves read node 0's entry
     int r[4];
@@ -1332,15 +1325,21 @@ Lib lib_of(gls_ctx *ctx) {
     return Lib{ctx->d_npn, ctx->d_perms4, ctx->d_lib_nodes, ctx->d_lib_node_off, ctx->d_lib_root_off, ctx->d_lib_roots};
 }
 
+// Stage timer of a pass (diagnostics only): an event call that fails makes
+// every stage time read 0 instead of failing the pass.
 struct Timer {
     gls_ctx *ctx;
     int next = 0;
+    bool ok = true;
     void mark() {
-        if (next < 16) hipEventRecord(ctx->ev[next++], ctx->stream);
+        if (next < 16) {
+            if (hipEventRecord(ctx->ev[next], ctx->stream) != hipSuccess) ok = false;
+            ++next;
+        }
     }
     double ms(int a, int b) {
         float x = 0;
-        if (b < next) hipEventElapsedTime(&x, ctx->ev[a], ctx->ev[b]);
+        if (!ok || b >= next || hipEventElapsedTime(&x, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0;
         return x;
     }
 };
