@@ -916,7 +916,9 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
     uint32_t w = L.npn[tt];
     int uPhase = w & 31, perm = (w >> 5) & 31, cls = (w >> 10) & 255;
     const uint8_t *pp = L.perms4 + 4 * perm;
-    for (int j = 0; j < NMATCH; j++) { match[j] = -1; mlev[j] = (LV)-1; }
+    // entries 0..3 (the inputs) and 4..nn+3 (the class's forest nodes) are all
+    // written below before they are read; the library's roots and children
+    // index only those, so the columns need no clearing
     isC = 0;
     for (int j = 0; j < 4; j++) {
         const int q = pp[j];
@@ -931,14 +933,18 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
         int a = wd & 63, b = (wd >> 6) & 63, ca = (wd >> 12) & 1, cb = (wd >> 13) & 1;
         int num = j + 4;
         const int la = (int)mlev[a], lb = (int)mlev[b];
-        mlev[num] = (LV)(1 + (la > lb ? la : lb));
-        int ma = match[a], mb = match[b];
-        if (ma == -1 || mb == -1 || ma == id || mb == id) continue;
-        int nid = table_lookup(g, ma, mb, (int)isc_bit(isC, a) ^ ca, (int)isc_bit(isC, b) ^ cb);
-        if (nid != -1 && !mffc_deleted(t, nid)) {
-            match[num] = nid;
-            mlev[num] = (LV)g.level[nid];
+        int lev = 1 + (la > lb ? la : lb);
+        const int ma = match[a], mb = match[b];
+        int found = -1;
+        if (!(ma == -1 || mb == -1 || ma == id || mb == id)) {
+            const int nid = table_lookup(g, ma, mb, (int)isc_bit(isC, a) ^ ca, (int)isc_bit(isC, b) ^ cb);
+            if (nid != -1 && !mffc_deleted(t, nid)) {
+                found = nid;
+                lev = (int)g.level[nid];
+            }
         }
+        match[num] = found;
+        mlev[num] = (LV)lev;
     }
     return cls;
 }
