@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--serve-max", type=int, default=2_000_000, help="largest AND count of a random job")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-max", type=int, default=200_000, help="largest AND count of the batch mix")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="batch: time only the batched pass (no one-at-a-time loop, no identity check); "
+                         "for counter runs")
     ap.add_argument("--batch-check", type=int, default=256,
                     help="members compared array by array with their one-at-a-time result")
     a = ap.parse_args()
@@ -281,6 +284,13 @@ def bench_batch(a):
         log("batch step %d: %.1f ms" % (i, times[-1] * 1e3))
     t_batch = sum(times) / len(times)
     info = out[0].last_op()
+    if a.no_loop:
+        print(json.dumps({"metric": "batched rw Mnodes/s, %d mixed AIGs" % len(aigs),
+                          "value": round(n_in / t_batch / 1e6, 4), "unit": "Mnodes/s",
+                          "ms_per_step": round(t_batch * 1e3, 3), "loop": None,
+                          "stages_ms": {k: round(info[k], 3) for k in ("cuts_ms", "eval_ms", "strash_ms",
+                                                                        "total_ms")}}), flush=True)
+        return
     # the plain loop: the same pass, one AIG at a time (each already on the device)
     c = copies()
     t = time.perf_counter()
