@@ -912,14 +912,16 @@ __device__ __forceinline__ int table_lookup(const GraphView &g, int in0, int in1
 // the same either way.
 template <typename LV>
 __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, const int *leaves,
-                         const Mffc &t, Col match, ColT<LV> mlev, uint32_t &isC) {
+                         const Mffc &t, Col match, ColT<LV> mlev, uint32_t &isC, uint64_t &matched) {
     uint32_t w = L.npn[tt];
     int uPhase = w & 31, perm = (w >> 5) & 31, cls = (w >> 10) & 255;
     const uint8_t *pp = L.perms4 + 4 * perm;
     // entries 0..3 (the inputs) and 4..nn+3 (the class's forest nodes) are all
     // written below before they are read; the library's roots and children
-    // index only those, so the columns need no clearing
+    // index only those, so the columns need no clearing.  matched: bit j set
+    // when entry j holds an existing node (the inputs always do)
     isC = 0;
+    matched = 0xFull;
     for (int j = 0; j < 4; j++) {
         const int q = pp[j];
         const int leaf = q == 0 ? leaves[0] : q == 1 ? leaves[1] : q == 2 ? leaves[2] : leaves[3];
@@ -945,16 +947,22 @@ __device__ int match_cut(const GraphView &g, const Lib &L, int id, uint32_t tt, 
         }
         match[num] = found;
         mlev[num] = (LV)lev;
+        if (found != -1) matched |= 1ull << num;
     }
     return cls;
 }
 
-__device__ __forceinline__ uint64_t added_set(const Lib &L, int cls, int rt, Col match) {
+// Forest nodes that replacing with root rt would create (Eval, rewrite.cu:369-374,
+// counts them): from rt down, every unmatched node that a created node uses.  Children have
+// lower indexes than their parents, so taking the highest unmatched used node
+// each time visits exactly the nodes of a descending scan over all indexes,
+// in the same order, with one step per created node instead of per index.
+__device__ __forceinline__ uint64_t added_set(const Lib &L, int cls, int rt, uint64_t matched) {
     uint64_t used = 1ull << rt, add = 0;
-    for (int j = rt; j >= 4; j--) {
-        if (!((used >> j) & 1) || match[j] > -1) continue;
+    for (uint64_t cand = used & ~matched; cand; cand = used & ~matched & ~add) {
+        const int j = 63 - __clzll((long long)cand);
         add |= 1ull << j;
-        uint32_t wd = lib_word(L, cls, j - 4);
+        const uint32_t wd = lib_word(L, cls, j - 4);
         used |= 1ull << (wd & 63);
         used |= 1ull << ((wd >> 6) & 63);
     }
@@ -992,12 +1000,13 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_eval(int32_t n, int32_t nPIs, Gr
         Mffc t;
         int saved = calc_mffc(g, id, leaves, nL, t);
         uint32_t isC;
-        int cls = match_cut(g, L, id, tt, leaves, t, match, mlev, isC);
+        uint64_t matched;
+        int cls = match_cut(g, L, id, tt, leaves, t, match, mlev, isC, matched);
         const int nr = lib_nr(L, cls);
         for (int out = 0; out < nr; out++) {
             int rt = lib_root(L, cls, out);
             if (match[rt] == id) continue;
-            int added = __popcll(added_set(L, cls, rt, match));
+            int added = __popcll(added_set(L, cls, rt, matched));
             int gain = saved - added;
             if (gain < 0 || (gain == 0 && !useZeros)) continue;
             int rl = (int)mlev[rt];
@@ -1047,9 +1056,10 @@ __device__ PlanCore plan_core(const GraphView &g, const Lib &L, int id, const ui
     cut_leaves4(p, leaves, nL, tt);
     calc_mffc(g, id, leaves, nL, t);
     PlanCore pc;
-    pc.cls = match_cut(g, L, id, tt, leaves, t, match, mlev, pc.isC);
+    uint64_t matched;
+    pc.cls = match_cut(g, L, id, tt, leaves, t, match, mlev, pc.isC, matched);
     pc.rt = lib_root(L, pc.cls, out);
-    pc.addMask = added_set(L, pc.cls, pc.rt, match);
+    pc.addMask = added_set(L, pc.cls, pc.rt, matched);
     pc.valid = 1;
     uint32_t w = L.npn[tt];
     uint32_t rtt = lib_truth(L, pc.cls, pc.rt, L.perms4 + 4 * ((w >> 5) & 31), pc.isC);
