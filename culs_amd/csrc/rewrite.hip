@@ -845,31 +845,27 @@ __device__ __forceinline__ int mffc_decrease(const GraphView &g, int x, const in
 // CalcMFFC (rewrite.cu:319-326) without recursion: an explicit DFS stack that
 // visits fanin0's subtree before decreasing fanin1, as the recursion does.
 // At most MFFC_TAB nodes are pushed besides the root (a push needs a new
-// table entry), so MFFC_TAB + 2 entries never overflow.
-__device__ int calc_mffc(const GraphView &g, int root, const int *leaves, int nL, Mffc &t) {
-    constexpr int S = MFFC_TAB + 2;
-    int stkNode[S], stkStage[S];
-#pragma unroll
-    for (int j = 0; j < S; j++) { stkNode[j] = 0; stkStage[j] = 0; }
-    stkNode[0] = root;
+// table entry), so MFFC_TAB + 2 entries never overflow.  The stack's nodes go
+// to the LDS column stk (the caller's match column, free until library
+// matching fills it), one access each; their stages (0, 1: fanin to visit
+// next, 2: done) are 2-bit fields of one register.
+__device__ int calc_mffc(const GraphView &g, int root, const int *leaves, int nL, Mffc &t, Col stk) {
+    stk[0] = root;
+    uint32_t stg = 0;
     int sp = 1, ans = 1;
     t.size = 0;
 #pragma unroll
     for (int i = 0; i < MFFC_TAB; i++) { t.id[i] = 0; t.num[i] = 0; }
     while (sp) {
-        int cur = 0, stage = 0;
-#pragma unroll
-        for (int j = 0; j < S; j++)
-            if (j == sp - 1) { cur = stkNode[j]; stage = stkStage[j]; }
+        const int top = sp - 1;
+        const int stage = (int)((stg >> (2 * top)) & 3u);
         if (stage < 2) {
-#pragma unroll
-            for (int j = 0; j < S; j++)
-                if (j == sp - 1) stkStage[j] = stage + 1;
+            stg += 1u << (2 * top);
+            const int cur = stk[top];
             const int x = (int)LID(stage == 0 ? g.f0[cur] : g.f1[cur]);
             if (mffc_decrease(g, x, leaves, nL, t) == 0) {
-#pragma unroll
-                for (int j = 0; j < S; j++)
-                    if (j == sp) { stkNode[j] = x; stkStage[j] = 0; }
+                stk[sp] = x;
+                stg &= ~(3u << (2 * sp));
                 sp++;
                 ans++;
             }
@@ -998,7 +994,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_eval(int32_t n, int32_t nPIs, Gr
         uint32_t tt;
         cut_leaves4(p, leaves, nL, tt);
         Mffc t;
-        int saved = calc_mffc(g, id, leaves, nL, t);
+        int saved = calc_mffc(g, id, leaves, nL, t, match);
         uint32_t isC;
         uint64_t matched;
         int cls = match_cut(g, L, id, tt, leaves, t, match, mlev, isC, matched);
@@ -1054,7 +1050,7 @@ __device__ PlanCore plan_core(const GraphView &g, const Lib &L, int id, const ui
     int leaves[4], nL;
     uint32_t tt;
     cut_leaves4(p, leaves, nL, tt);
-    calc_mffc(g, id, leaves, nL, t);
+    calc_mffc(g, id, leaves, nL, t, match);
     PlanCore pc;
     uint64_t matched;
     pc.cls = match_cut(g, L, id, tt, leaves, t, match, mlev, pc.isC, matched);
