@@ -31,10 +31,14 @@ B_RW_PER_AND = 3 * K_CUTS * CUT_BYTES + 64
 # cut enumeration per AND: two fanin cut sets read, its own written (3*K*s_c),
 # plus its two fanin literals and its reference count (12 B)
 B_CUT_PER_AND = 3 * K_CUTS * CUT_BYTES + 12
-# HBM bytes per k_cuts_level launch on this workload (FETCH_SIZE + WRITE_SIZE,
-# rocprofv3 --pmc, scaled by the counters' reading for a plain copy kernel);
-# None until measured
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per k_cuts_triple launch on the default 10M workload and schedule:
+# FETCH_SIZE of a rocprofv3 --pmc run over all 12,913 launches, 430.7 KiB per
+# launch, x 2.0 (the counter reads half of a plain 256 MiB copy's bytes) =
+# 882,074 B; WRITE_SIZE, 160.0 B per node on a 200k-AND AIG of the same level
+# shape (the same counter reads exactly a plain copy's bytes), x 617.3 nodes
+# per launch = 98,770 B.  Reads are about 2.9x the algorithmic bytes because a
+# node is recomputed in the cones of its consumers within the launch.
+TRAFFIC_BYTES_PER_LAUNCH = 980_844
 
 
 def cut_kernel_name(mode):
@@ -182,8 +186,12 @@ def main():
         achieved = knodes * B_CUT_PER_AND / (kms / 1e3) / 1e9
     else:
         avg_launch_s, bytes_per_launch, achieved = 0.0, n_in * B_CUT_PER_AND / nlev, 0.0
+    # the measured traffic holds for the default workload and schedule only
+    default_workload = (a.cut_mode == 0 and a.nands == 10_000_000 and a.pis == 1024 and a.pos == 4096 and
+                        a.window == 50_000 and a.p == 0.001 and a.seed == 3)
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+                "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_workload else None,
                 "kernel": cut_kernel_name(a.cut_mode), "launches_per_step": cut_launches(a.cut_mode, nlev),
                 "timed_launches": ks,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
