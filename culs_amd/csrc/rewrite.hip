@@ -263,9 +263,11 @@ struct Slot {
 };
 
 
+// Leaf signature of a merged or support-minimized cut: its leaf list is padded
+// with its first leaf, so a padded position adds no bit; an empty cut has none.
 __device__ __forceinline__ uint32_t sig4(uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3, int nn) {
-    return (nn > 0 ? 1u << (n0 & 31) : 0u) | (nn > 1 ? 1u << (n1 & 31) : 0u) | (nn > 2 ? 1u << (n2 & 31) : 0u) |
-           (nn > 3 ? 1u << (n3 & 31) : 0u);
+    const uint32_t s = (1u << (n0 & 31)) | (1u << (n1 & 31)) | (1u << (n2 & 31)) | (1u << (n3 & 31));
+    return nn > 0 ? s : 0u;
 }
 
 // CutFilter (rewrite.cu:178-193) of the cut in slot `id` (leaves N, nn) against
