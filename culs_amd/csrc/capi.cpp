@@ -333,8 +333,7 @@ extern "C" void gls_aig_free(gls_aig *a) {
     delete a;
 }
 
-extern "C" int gls_aig_read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len, gls_aig **out) {
-    if (!ctx || !buf || !out) return GLS_ERR_ARG;
+static int read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len, gls_aig **out) {
     std::vector<uint32_t> f0, f1, pos;
     int32_t nPIs = 0;
     std::string err;
@@ -345,14 +344,29 @@ extern "C" int gls_aig_read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len
     return GLS_OK;
 }
 
+// Host allocation failures (a huge but well-formed file) come back as
+// GLS_ERR_NOMEM; no exception crosses the C ABI.
+extern "C" int gls_aig_read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len, gls_aig **out) {
+    if (!ctx || !buf || !out) return GLS_ERR_ARG;
+    try {
+        return read_aiger_mem(ctx, buf, len, out);
+    } catch (const std::bad_alloc &) {
+        return gls_fail(ctx, GLS_ERR_NOMEM, "out of host memory reading an AIGER buffer of %lld bytes", (long long)len);
+    }
+}
+
 extern "C" int gls_aig_read_aiger(gls_ctx *ctx, const char *path, gls_aig **out) {
     if (!ctx || !path || !out) return GLS_ERR_ARG;
-    std::ifstream in(path, std::ios::binary);
-    if (!in) return gls_fail(ctx, GLS_ERR_IO, "cannot open %s", path);
-    std::stringstream ss;
-    ss << in.rdbuf();
-    std::string s = ss.str();
-    return gls_aig_read_aiger_mem(ctx, s.data(), (int64_t)s.size(), out);
+    try {
+        std::ifstream in(path, std::ios::binary);
+        if (!in) return gls_fail(ctx, GLS_ERR_IO, "cannot open %s", path);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        std::string s = ss.str();
+        return read_aiger_mem(ctx, s.data(), (int64_t)s.size(), out);
+    } catch (const std::bad_alloc &) {
+        return gls_fail(ctx, GLS_ERR_NOMEM, "out of host memory reading %s", path);
+    }
 }
 
 extern "C" int gls_aig_write_aiger(gls_aig *a, const char *path) {

@@ -93,11 +93,21 @@ int aiger_parse(const char *buf, int64_t len, std::vector<uint32_t> &f0, std::ve
     if (L != 0) { err = "latches are not supported"; return GLS_ERR_FORMAT; }
     if (M != I + A) { err = "M != I + A"; return GLS_ERR_FORMAT; }
     if (M + 1 > 0x7FFFFFFF) { err = "AIG too large"; return GLS_ERR_FORMAT; }
+    // Every output line takes at least 2 bytes ("0\n") and every AND at least
+    // 2 (two one-byte deltas), so a header that promises more than the bytes
+    // left is rejected before anything is allocated.
+    const int64_t left = (int64_t)(end - p);
+    if (O > left / 2 || A > (left - 2 * O) / 2) { err = "AIGER header promises more outputs/ANDs than the file holds"; return GLS_ERR_FORMAT; }
     nPIs = (int32_t)I;
     const int64_t nObjs = M + 1;
-    f0.assign(nObjs, 0);
-    f1.assign(nObjs, 0);
-    pos.assign(O, 0);
+    try {
+        f0.assign(nObjs, 0);
+        f1.assign(nObjs, 0);
+        pos.assign(O, 0);
+    } catch (const std::bad_alloc &) {
+        err = "out of host memory for the AIG arrays";
+        return GLS_ERR_NOMEM;
+    }
     for (int64_t i = 0; i < O; i++) {
         int64_t x;
         if (!read_uint(p, end, x) || x > 2 * M + 1) { err = "bad output literal"; return GLS_ERR_FORMAT; }

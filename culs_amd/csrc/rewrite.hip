@@ -1327,6 +1327,17 @@ struct Tmp {
     }
 };
 
+// Owns a pool-backed graph until it is freed explicitly or the scope ends.
+struct GraphGuard {
+    gls_ctx *ctx;
+    DevGraph *g;
+    void free_now() {
+        if (g) dev_graph_free(ctx, *g);
+        g = nullptr;
+    }
+    ~GraphGuard() { free_now(); }
+};
+
 int blocks(int64_t n, int b) { return (int)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
 
 uint32_t bits_for(uint32_t x) {
@@ -1426,10 +1437,11 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
     if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: allocation");
     uint32_t maxLevel = 0;
     GLS_TRY(dev_graph_alloc(ctx, out, n, in.nPIs, in.nPOs));
+    GraphGuard outGuard{ctx, &out};   // released to the caller on success
     uint32_t *tags = nullptr;
     if (tagIn && tagOut) {
         tags = pool_get<uint32_t>(ctx, (size_t)n);
-        if (!tags) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: tags"); }
+        if (!tags) return gls_fail(ctx, GLS_ERR_NOMEM, "reorder: tags");
         *tagOut = tags;
     }
     if (m > 0) {
@@ -1472,6 +1484,7 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
         GLS_HIP(ctx, hipStreamSynchronize(st));
         levelStart.assign(2, n);
     }
+    outGuard.g = nullptr;
     return GLS_OK;
 }
 
@@ -1741,6 +1754,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         ~TagGuard() { if (*p) pool_put(c, *p); }
     } tagGuard{ctx, &tagR};
     GLS_TRY(dev_reorder(ctx, in, g, levelStart, tagged ? tagIn : nullptr, tagged ? &tagR : nullptr));
+    GraphGuard gGuard{ctx, &g};   // frees the reordered graph on every return path
     tm.mark(); // 1
     const int32_t n = g.nObjs, nPIs = g.nPIs, nPOs = g.nPOs;
     int rc = GLS_OK;
@@ -1753,12 +1767,12 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         uint32_t *candIds = t.get<uint32_t>(n), *d_cnt = t.get<uint32_t>(4), *d_err = t.get<uint32_t>(4);
         uint32_t *repl = t.get<uint32_t>(n);
         unsigned long long *d_gain = t.get<unsigned long long>(1);
-        if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n); }
+        if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: allocation (n=%d)", n);
         GLS_HIP(ctx, hipMemsetAsync(d_err, 0, 16, st));
         uint8_t *sigBits = nullptr;
         if (tagged) {
             sigBits = t.get<uint8_t>(n);
-            if (!sigBits) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: signature bits"); }
+            if (!sigBits) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: signature bits");
             GLS_TRY(sig_bits_for_tags(ctx, n, tagR, sigBits));
         }
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
@@ -1806,7 +1820,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             newOff = t.get<uint32_t>(nCand + 1);
             Claims cl{t.get<unsigned long long>(n), t.get<unsigned long long>(n), t.get<unsigned long long>(n),
                       t.get<unsigned long long>(n)};
-            if (!t.ok) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: plan allocation"); }
+            if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: plan allocation");
             k_plan<<<blocks(nCand, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>((int32_t)nCand, candIds, gv, L, cuts, best, plans);
             k_sel_init<<<blocks(nCand, kBlock), kBlock, 0, st>>>((int32_t)nCand, plans, state);
             GLS_HIP(ctx, hipMemsetAsync(mk, 0, sizeof(uint32_t) * n, st));
@@ -1825,7 +1839,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
                 GLS_HIP(ctx, hipStreamSynchronize(st));
                 rounds = r;
                 if (left == 0) break;
-                if (r > 60000) { dev_graph_free(ctx, g); return gls_fail(ctx, GLS_ERR_INTERNAL, "selection did not converge"); }
+                if (r > 60000) return gls_fail(ctx, GLS_ERR_INTERNAL, "selection did not converge");
             }
             GLS_HIP(ctx, hipMemsetAsync(d_gain, 0, 8, st));
             k_new_counts<<<blocks(nCand, kBlock), kBlock, 0, st>>>((int32_t)nCand, plans, state, newCnt, d_gain);
@@ -1847,10 +1861,11 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         const int32_t N = n + (int32_t)totalNew;
         DevGraph comb;
         GLS_TRY(dev_graph_alloc(ctx, comb, N, nPIs, nPOs));
+        GraphGuard combGuard{ctx, &comb};
         uint32_t *tagC = nullptr;   // tags of the composed graph
         if (tagged) {
             tagC = t.get<uint32_t>(N);
-            if (!tagC) { dev_graph_free(ctx, g); dev_graph_free(ctx, comb); return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: tags"); }
+            if (!tagC) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: tags");
             GLS_HIP(ctx, hipMemcpyAsync(tagC, tagR, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
         }
         if (nCand > 0 && totalNew + nSel > 0)
@@ -1862,11 +1877,11 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         GLS_HIP(ctx, hipGetLastError());
         GLS_HIP(ctx, hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, st));
         GLS_HIP(ctx, hipStreamSynchronize(st));
-        if (herr) { dev_graph_free(ctx, g); dev_graph_free(ctx, comb); return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr); }
+        if (herr) return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr);
         tm.mark(); // 6
-        dev_graph_free(ctx, g);
+        gGuard.free_now();
         rc = dev_strash(ctx, comb, out, depth, nullptr, tagC, tagged ? tagOut : nullptr, tagged ? tagDepth : nullptr);
-        dev_graph_free(ctx, comb);
+        combGuard.free_now();
         tm.mark(); // 7
         GLS_HIP(ctx, hipStreamSynchronize(st));
         if (info) {

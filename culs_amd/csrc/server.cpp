@@ -68,7 +68,10 @@ struct gls_job {
     int32_t nPIs = 0;
     std::vector<uint32_t> f0, f1, pos;    // input until admitted, result when done (AIGER literals)
     gls_aig *aig = nullptr;
-    gls_job_info info = {};
+    gls_job_info info = {};   // read by clients: written only under gls_server::mu
+    // worker-only state, published into info under the lock when the job ends
+    int32_t wstatus = 1;
+    gls_stats wres = {};
 };
 
 struct gls_server {
@@ -117,7 +120,7 @@ void gls_server::run() {
             int rc = gls_aig_from_arrays(ctx, j->nPIs, (int32_t)j->pos.size(), nAnds, j->f0.data(), j->f1.data(),
                                          j->pos.empty() ? nullptr : j->pos.data(), GLS_LIT_AIGER, &j->aig);
             if (rc == GLS_OK) rc = aig_ensure_device(j->aig);
-            if (rc != GLS_OK) { j->info.status = rc; j->pc = j->cmds.size(); }
+            if (rc != GLS_OK) { j->wstatus = rc; j->pc = j->cmds.size(); }
             std::vector<uint32_t>().swap(j->f0);
             std::vector<uint32_t>().swap(j->f1);
         }
@@ -125,7 +128,7 @@ void gls_server::run() {
         std::vector<gls_aig *> group[CMD_KINDS];
         std::vector<gls_job *> owners[CMD_KINDS];
         for (auto &j : active) {
-            if (j->pc >= j->cmds.size() || j->info.status < 0) continue;
+            if (j->pc >= j->cmds.size() || j->wstatus < 0) continue;
             group[j->cmds[j->pc]].push_back(j->aig);
             owners[j->cmds[j->pc]].push_back(j.get());
         }
@@ -137,7 +140,7 @@ void gls_server::run() {
                      : k == CMD_RWZ ? gls_batch_rewrite(ctx, group[k].data(), cnt, 1, 0)
                                     : gls_batch_balance(ctx, group[k].data(), cnt, 1, 1);
             for (gls_job *j : owners[k]) {
-                if (rc != GLS_OK) { j->info.status = rc; j->pc = j->cmds.size(); }
+                if (rc != GLS_OK) { j->wstatus = rc; j->pc = j->cmds.size(); }
                 else j->pc++;
             }
             if (rc != GLS_OK) {
@@ -149,19 +152,17 @@ void gls_server::run() {
         std::vector<std::shared_ptr<gls_job>> finished;
         for (auto &j : active)
             if (j->pc >= j->cmds.size()) finished.push_back(j);
+        // the result arrays are filled before the job is published as done;
+        // clients read them only after seeing status 2 under the lock
         for (auto &j : finished) {
-            if (j->info.status >= 0 && j->aig) {
-                gls_stats s;
+            if (j->wstatus >= 0 && j->aig) {
+                gls_stats &s = j->wres;
                 gls_aig_stats(j->aig, &s);
                 j->f0.resize(s.nObjs);
                 j->f1.resize(s.nObjs);
                 j->pos.resize(s.nPOs);
                 int rc = gls_aig_to_arrays(j->aig, j->f0.data(), j->f1.data(), j->pos.data(), GLS_LIT_AIGER);
-                j->info.nPIs = s.nPIs;
-                j->info.nPOs = s.nPOs;
-                j->info.nAnds = s.nAnds;
-                j->info.levels = s.levels;
-                j->info.status = rc == GLS_OK ? 2 : rc;
+                j->wstatus = rc == GLS_OK ? 2 : rc;
             }
             if (j->aig) gls_aig_free(j->aig);
             j->aig = nullptr;
@@ -169,6 +170,13 @@ void gls_server::run() {
         {
             std::lock_guard<std::mutex> lk(mu);
             for (auto &j : finished) {
+                if (j->wstatus == 2) {
+                    j->info.nPIs = j->wres.nPIs;
+                    j->info.nPOs = j->wres.nPOs;
+                    j->info.nAnds = j->wres.nAnds;
+                    j->info.levels = j->wres.levels;
+                }
+                j->info.status = j->wstatus;
                 j->info.done_ms = now_ms();
                 used -= j->info.footprint_bytes;
                 active.erase(std::find(active.begin(), active.end(), j));
@@ -290,10 +298,14 @@ extern "C" int gls_server_release(gls_server *s, int64_t job) {
     return GLS_OK;
 }
 
+// The text is copied per calling thread, so the worker replacing the
+// server's message cannot invalidate what a client holds.
 extern "C" const char *gls_server_last_error(gls_server *s) {
+    static thread_local std::string copy;
     if (!s) return "";
     std::lock_guard<std::mutex> lk(s->mu);
-    return s->err.c_str();
+    copy = s->err;
+    return copy.c_str();
 }
 
 extern "C" void gls_server_destroy(gls_server *s) {

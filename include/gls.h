@@ -158,7 +158,8 @@ int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_
  * every command kind once as a batched pass over the active jobs whose next
  * command it is, so jobs that arrive while others run join the next round.
  * A job's result equals running its script alone.  All functions are
- * thread-safe; gls_server_last_error's text is valid until the next call. */
+ * thread-safe; gls_server_last_error's text is valid until the same thread's
+ * next call to it. */
 typedef struct gls_server gls_server;
 typedef struct {
     int32_t status;                 /* 0 queued, 1 on the device, 2 done, < 0 error code */

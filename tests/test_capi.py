@@ -79,10 +79,15 @@ def test_aiger_hand_written_file(host_ctx):
 
 
 @pytest.mark.parametrize("bad", [b"aag 3 2 0 1 1\n", b"aig 3 2 1 1 1\n", b"aig 4 2 0 1 1\n7\n\x01\x03",
-                                 b"aig 3 2 0 1 1\n7\n", b"aig 3 2 0 1 1 0\n7\n\x01\x03"])
+                                 b"aig 3 2 0 1 1\n7\n", b"aig 3 2 0 1 1 0\n7\n\x01\x03",
+                                 # headers that promise far more than the file holds: rejected
+                                 # before any allocation (terabytes of outputs / 16 GB of ANDs)
+                                 b"aig 2147483646 0 0 1099511627775 2147483646\n",
+                                 b"aig 2147483646 0 0 0 2147483646\n\x01\x01"])
 def test_aiger_rejects_malformed(host_ctx, bad):
-    with pytest.raises(GlsError):
+    with pytest.raises(GlsError) as e:
         host_ctx.read_aiger_bytes(bad)
+    assert "(-4)" in str(e.value)
 
 
 def test_from_arrays_validates_topological_order(host_ctx):

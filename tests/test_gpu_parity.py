@@ -25,6 +25,9 @@ CASES = [
     ("mult8_red", dict(kind="mult", nbits=8, redundant=0.1, seed=2)),
     ("cfg1_mult32_red", dict(kind="mult", nbits=32, redundant=0.1, seed=2)),
     ("geom_50k", dict(kind="geom", nPIs=128, nAnds=50000, nPOs=256, window=5000, p=0.01, seed=3)),
+    # the 10M-AND workload's generator and parameters (1,024 PIs, 4,096 POs,
+    # window 50,000, p = 0.001, seed 3) at 200k ANDs
+    ("cfg2_shape_200k", dict(kind="geom", nPIs=1024, nAnds=200000, nPOs=4096, window=50000, p=0.001, seed=3)),
 ]
 
 
@@ -88,6 +91,10 @@ def test_rewrite_parity(gpu_ctx, name, spec):
         info = a.last_op()
         assert info["candidates"] == st[0] and info["selected"] == st[1] and info["new_nodes"] == st[2]
         assert same(got, want), "GPU rw (zeros=%s) differs from the oracle" % z
+        # the selected replacements are disjoint, so the pass removes at least
+        # the sum of their gains (the final strash can only remove more)
+        assert info["est_gain"] == st[6]
+        assert O.n_ands(s) - O.n_ands(got) >= info["est_gain"]
         assert O.sim_cmp(g, got) == -1, "rw output not equivalent to the input"
         s = got
 
