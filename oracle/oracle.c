@@ -624,7 +624,11 @@ int orc_reorder(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPO
     for (int l = 1; l <= maxL; l++) cnt[l] += cnt[l - 1];
     int32_t *order = malloc(sizeof(int32_t) * (size_t)n);
     order[0] = 0;
+#ifndef ORC_MUT_REORDER
     for (int v = n - 1; v >= 1; v--) order[v] = cnt[lev[v]]--;
+#else
+    for (int v = 1; v < n; v++) order[v] = cnt[lev[v]]--;
+#endif
     free(cnt);
     for (int v = 0; v <= nPIs; v++) { of0[v] = 0; of1[v] = 0; }
     for (int v = nPIs + 1; v < n; v++) {
@@ -641,6 +645,9 @@ int orc_reorder(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPO
 
 /* ------------------------------------------------------------------ */
 /* Cut enumeration (rewrite.cu:49-294)                                 */
+/* (ORC_MUT_* macros plant one mistake each for tests/test_oracle_     */
+/* mutants.py, which checks that the pins catch it; never defined in   */
+/* the normal build.)                                                  */
 /* ------------------------------------------------------------------ */
 typedef struct { uint32_t leaves[4]; uint32_t tt, value; int nLeaves, used; } Cut;
 
@@ -664,7 +671,9 @@ static uint32_t cut_value(const Cut *c, const int32_t *nRef) {     /* rewrite.cu
     }
     if (c->nLeaves < 2) return 1001;
     if (value > 1000) value = 1000;
+#ifndef ORC_MUT_VALUE
     if (nOnes > 3) value = 5 - nOnes;
+#endif
     return (uint32_t)value;
 }
 
@@ -672,7 +681,11 @@ static int find_cut(Cut *cs) {                                    /* rewrite.cu:
     for (int i = 0; i < NCUT; i++) if (!cs[i].used) return i;
     int ans = -1;
     for (int i = 0; i < NCUT; i++)
+#ifndef ORC_MUT_FINDCUT
         if (cs[i].nLeaves > 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
+#else
+        if (cs[i].nLeaves > 2 && (ans == -1 || cs[i].value <= cs[ans].value)) ans = i;
+#endif
     if (ans == -1)
         for (int i = 0; i < NCUT; i++)
             if (cs[i].nLeaves == 2 && (ans == -1 || cs[i].value < cs[ans].value)) ans = i;
@@ -732,7 +745,11 @@ static int cut_filter(Cut *cs, int id) {                          /* rewrite.cu:
             if (dominates(&cs[id], &cs[i])) cs[i].used = 0;
         } else {
             if ((si & sid) != si) continue;
+#ifndef ORC_MUT_DOMINANCE
             if (dominates(&cs[i], &cs[id])) { cs[id].used = 0; return 1; }
+#else
+            if (dominates(&cs[id], &cs[i])) { cs[id].used = 0; return 1; }
+#endif
         }
     }
     return 0;
@@ -953,7 +970,11 @@ static int match_cut(const Graph *g, int id, const Cut *cut, const int *leaves, 
 static uint64_t added_set(int cls, int rt, const int32_t *match) {
     uint64_t used = 1ull << rt, add = 0;
     for (int j = rt; j >= 4; j--) {
+#ifndef ORC_MUT_ADDED
         if (!((used >> j) & 1) || match[j] > -1) continue;
+#else
+        if (!((used >> j) & 1)) continue;
+#endif
         add |= 1ull << j;
         used |= 1ull << lib_in0(cls, j - 4);
         used |= 1ull << lib_in1(cls, j - 4);
@@ -984,7 +1005,11 @@ static void eval_node(const Graph *g, int id, const Cut *cs, int useZeros, int32
             if (match[rt] == id) continue;
             int added = __builtin_popcountll(added_set(cls, rt, match));
             int gain = saved - added;
+#ifndef ORC_MUT_ZERO
             if (gain < 0 || (gain == 0 && !useZeros)) continue;
+#else
+            if (gain < 0) continue;
+#endif
             if (gain < reduction || (gain == reduction && mlev[rt] >= bestLevel)) continue;
             reduction = gain; bestLevel = mlev[rt]; bestCut = i; bestOut = out;
         }

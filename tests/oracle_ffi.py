@@ -3,6 +3,7 @@
 Only the tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
 and only as the checker.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -26,19 +27,35 @@ def lib():
     if _lib is None:
         if not os.path.exists(ORACLE_LIB):
             build_oracle()
-        L = ctypes.CDLL(ORACLE_LIB)
-        i32, u64 = ctypes.c_int, ctypes.c_uint64
-        L.orc_npn_tables.argtypes = [_P] * 4
-        L.orc_strash.argtypes = [i32, i32, _P, _P, i32, _P, _P, _P, _P, _P]
-        L.orc_reorder.argtypes = [i32, i32, _P, _P, i32, _P, _P, _P, _P, _P]
-        L.orc_cuts.argtypes = [i32, i32, _P, _P, i32, _P, _P]
-        L.orc_eval.argtypes = [i32, i32, _P, _P, i32, _P, _P, i32, _P]
-        L.orc_rw.argtypes = [i32, i32, _P, _P, i32, _P, i32, i32, _P, _P, _P, _P]
-        L.orc_rw_culs.argtypes = [i32, i32, _P, _P, i32, _P, i32, i32, _P, _P, _P, _P]
-        L.orc_sim_cmp.argtypes = [i32, _P, _P, _P, i32, _P, _P, _P, i32, i32, i32, u64]
-        L.orc_depth.argtypes = [i32, i32, _P, _P]
-        _lib = L
+        _lib = _load(ORACLE_LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Runs the wrappers against another build of the oracle (a mutant)."""
+    global _lib
+    old = lib()
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = old
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    i32, u64 = ctypes.c_int, ctypes.c_uint64
+    L.orc_npn_tables.argtypes = [_P] * 4
+    L.orc_strash.argtypes = [i32, i32, _P, _P, i32, _P, _P, _P, _P, _P]
+    L.orc_reorder.argtypes = [i32, i32, _P, _P, i32, _P, _P, _P, _P, _P]
+    L.orc_cuts.argtypes = [i32, i32, _P, _P, i32, _P, _P]
+    L.orc_eval.argtypes = [i32, i32, _P, _P, i32, _P, _P, i32, _P]
+    L.orc_rw.argtypes = [i32, i32, _P, _P, i32, _P, i32, i32, _P, _P, _P, _P]
+    L.orc_rw_culs.argtypes = [i32, i32, _P, _P, i32, _P, i32, i32, _P, _P, _P, _P]
+    L.orc_sim_cmp.argtypes = [i32, _P, _P, _P, i32, _P, _P, _P, i32, i32, i32, u64]
+    L.orc_depth.argtypes = [i32, i32, _P, _P]
+    return L
 
 
 def P(a):
