@@ -8,9 +8,14 @@ value = ANDs entering rw / seconds per step.
 Also printed in the same JSON line:
   roofline      the dominant kernel's algorithmic bytes per launch / its average
                 launch time (HIP events on the context stream), vs 8 TB/s HBM;
-  cpu_baseline  the CPU oracle's rw ("port", 1 core) on a bounded sample of the
-                same generator, in the same unit;
-  qor           AND counts, depth, and a simulation equivalence check.
+  cpu_baseline  the CPU oracle's sequential rewriter that follows the
+                reference's replacement method ("port", the ABC-proxy), built
+                -O3 -march=native on the host, pinned to one core, median of 5,
+                with the oracle of this package's pass beside it;
+  qor           AND counts, depth, and equivalence: the library's device check
+                (gls_cec, 4,096 patterns) and the oracle's host simulation;
+  e2e           `strash; rw` end to end on the input AND count;
+  hash          strash-table probes and probes per second (device counters).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nands N] [--cpu-sample N] [--cut-mode M]
        [--stage rw|cuts|strash] [--workload rw10m|batch|serve]
@@ -39,6 +44,9 @@ B_CUT_PER_AND = 3 * K_CUTS * CUT_BYTES + 12
 # per launch = 98,770 B.  Reads are about 2.9x the algorithmic bytes because a
 # node is recomputed in the cones of its consumers within the launch.
 TRAFFIC_BYTES_PER_LAUNCH = 980_844
+TRAFFIC_NOTE = ("profiled constant of the three-level cut kernel (profiles/k_cuts_triple_rw10m_traffic.md): "
+                "FETCH_SIZE measured over the 10M pass; the WRITE_SIZE half (98,770 B) is extrapolated from a "
+                "200k-AND AIG of the same level shape at 160.0 B per node")
 
 
 def cut_kernel_name(mode):
@@ -78,6 +86,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) 5 times, to scale HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline runs (median)")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
                     help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
@@ -123,9 +132,10 @@ def main():
     if not ctx.has_device:
         raise SystemExit("no GPU")
     t0 = time.time()
-    src = ctx.gen_geometric(a.pis, a.nands, a.pos, a.window, a.p, a.seed)
-    g_in = src.arrays() if not a.no_check else None
+    raw = ctx.gen_geometric(a.pis, a.nands, a.pos, a.window, a.p, a.seed)
+    g_in = raw.arrays() if not a.no_check else None
     log("generated %d ANDs in %.1fs" % (a.nands, time.time() - t0))
+    src = raw.copy()
     src.strash()
     st_info = src.last_op()
     s = src.stats()
@@ -133,6 +143,15 @@ def main():
     log("strash: %d -> %d ANDs, depth %d, %.1f ms" % (a.nands, n_in, s.levels, st_info["total_ms"]))
     if a.stage == "strash":
         return
+    # the input strash again (device time of gls_strash, the upload excluded), for `strash; rw` end to end
+    st_runs = [st_info]
+    for _ in range(2):
+        c = raw.copy()
+        c.strash()
+        st_runs.append(c.last_op())
+        c.free()
+    strash_ms = sorted(r["total_ms"] for r in st_runs)[len(st_runs) // 2]
+    strash_probes = st_runs[-1]["hash_probes"]
 
     def step():
         c = src.copy()
@@ -191,7 +210,12 @@ def main():
                         a.window == 50_000 and a.p == 0.001 and a.seed == 3)
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "achieved_basis": "algorithmic bytes (%d B per node) / measured launch time" % B_CUT_PER_AND,
+                "limiter": "instruction issue along each node's dependent chain and the launch latency of "
+                           "the depth-long launch sequence, not HBM bandwidth (counter profile: "
+                           "profiles/k_cuts_triple_geom200k_counters.md)",
                 "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_workload else None,
+                "traffic_note": TRAFFIC_NOTE if default_workload else None,
                 "kernel": cut_kernel_name(a.cut_mode), "launches_per_step": cut_launches(a.cut_mode, nlev),
                 "timed_launches": ks,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
@@ -200,26 +224,35 @@ def main():
 
     qor = {"and_in": a.nands, "and_strashed": n_in, "and_out": n_out, "depth_in": s.levels, "depth_out": depth_out}
     if g_in is not None:
+        # the library's device check (gls_cec) on 4,096 patterns, and the
+        # oracle's host simulation on 128 patterns as an independent check
+        t = time.time()
+        ci = raw.cec(out, words=64, seed=11)
+        qor["cec"] = {"equivalent": bool(ci["equivalent"]), "patterns": ci["patterns"],
+                      "method": "gls_cec: device bit-parallel simulation, input vs rw output",
+                      "s": round(time.time() - t, 3)}
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_ffi as O
         t = time.time()
         qor["sim_equivalent"] = bool(O.sim_cmp(g_in, out.arrays(), words=2, seed=11) == -1)
-        log("equivalence check %.1fs" % (time.time() - t))
+        log("equivalence checks: %s, oracle %.1fs" % (qor["cec"], time.time() - t))
 
     cpu = None
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_ffi as O
         ns = a.cpu_sample if a.cpu_sample > 0 else a.nands
-        sample = ctx.gen_geometric(a.pis, ns, a.pos, a.window, a.p, a.seed).arrays()
-        ss, _ = O.strash(*sample)
-        t = time.perf_counter()
-        O.rw(*ss)
-        dt = time.perf_counter() - t
-        cpu = {"value": round(O.n_ands(ss) / dt / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
-               "sample": "oracle rw (C, one thread) on the same generator and seed at %d ANDs "
-                         "(%d after strash), %.1f s" % (ns, O.n_ands(ss), dt)}
+        sample = g_in if (g_in is not None and ns == a.nands) else \
+            ctx.gen_geometric(a.pis, ns, a.pos, a.window, a.p, a.seed).arrays()
+        cpu = cpu_baseline(sample, ns, a.cpu_runs)
         log("cpu baseline: %s" % cpu)
+
+    e2e_ms = strash_ms + ms_per_step
+    hash_info = {"strash_probes": int(strash_probes),
+                 "strash_probes_per_s": round(strash_probes / (strash_ms / 1e3), 1) if strash_ms > 0 else None,
+                 "rw_probes": int(info["hash_probes"]),
+                 "rw_probes_per_s": round(info["hash_probes"] / (info["total_ms"] / 1e3), 1)
+                 if info["total_ms"] > 0 else None,
+                 "probe": "one compare-and-swap on a strash table key slot, or one value update through a "
+                          "pre-inserted key's stored location (device counters in the sweep kernels)"}
 
     res = {
         "metric": "rw Mnodes/s on 10M-AND AIG",
@@ -244,9 +277,97 @@ def main():
         "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
                                                       "build_ms", "strash_ms", "total_ms")},
         "rw_counts": {k: info[k] for k in ("candidates", "selected", "new_nodes", "rounds", "est_gain")},
-        "strash_ms": round(st_info["total_ms"], 3),
+        "strash_ms": round(strash_ms, 3),
+        "e2e": {"metric": "strash; rw Mnodes/s on the input ANDs", "value": round(a.nands / e2e_ms / 1e3, 4),
+                "unit": "Mnodes/s", "strash_ms": round(strash_ms, 3), "rw_ms": round(ms_per_step, 3)},
+        "hash": hash_info,
     }
+    if cpu is not None and cpu.get("e2e_value"):
+        res["e2e"]["vs_cpu"] = round(res["e2e"]["value"] / cpu["e2e_value"], 3)
+    if cpu is not None:
+        res["vs_cpu"] = round(value / cpu["value"], 3)
     print(json.dumps(res), flush=True)
+
+
+def build_native_oracle():
+    """The oracle compiled for this host (-O3 -march=native) into a temporary
+    directory; None if gcc is not usable here."""
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="gls_cpu_")
+    path = os.path.join(d, "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-shared", "-std=c11", "-o", path,
+           os.path.join(ROOT, "oracle", "oracle.c")]
+    try:
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120)
+    except (OSError, subprocess.SubprocessError):
+        return None, None
+    return path, " ".join(cmd[:5])
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sample, n_raw, runs):
+    """CPU rewriting on the host's own cores: the oracle's sequential
+    rewriter that follows the reference's own replacement method
+    (orc_rw_culs, rewrite.cu:811-938 — the ABC-proxy of the CPU baseline) is
+    `value`; the oracle's restatement of this package's pass (orc_rw) runs
+    beside it.  One thread pinned to one core, median of `runs` runs, the
+    oracle built -O3 -march=native for this host; the input strash is timed
+    the same way for `strash; rw` end to end."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import statistics
+    import oracle_ffi as O
+    path, flags = build_native_oracle()
+    core = None
+    old_aff = None
+    if hasattr(os, "sched_getaffinity"):
+        old_aff = os.sched_getaffinity(0)
+        core = min(old_aff)
+        os.sched_setaffinity(0, {core})
+    try:
+        import contextlib
+        with (O.using(path) if path else contextlib.nullcontext()):
+            t_st, t_culs, t_rw = [], [], []
+            ss = None
+            for _ in range(runs):
+                t = time.perf_counter()
+                ss, _ = O.strash(*sample)
+                t_st.append(time.perf_counter() - t)
+            for _ in range(runs):
+                t = time.perf_counter()
+                rc, _ = O.rw_culs(*ss)
+                t_culs.append(time.perf_counter() - t)
+            for _ in range(max(1, (runs + 1) // 2)):
+                t = time.perf_counter()
+                rp, _ = O.rw(*ss)
+                t_rw.append(time.perf_counter() - t)
+    finally:
+        if old_aff is not None:
+            os.sched_setaffinity(0, old_aff)
+    n = O.n_ands(ss)
+    m_st, m_culs, m_rw = statistics.median(t_st), statistics.median(t_culs), statistics.median(t_rw)
+    return {"value": round(n / m_culs / 1e6, 4), "unit": "Mnodes/s", "cores": 1, "kind": "port",
+            "sample": "the workload's generator and seed at %d ANDs (%d after strash); oracle orc_rw_culs (the reference's "
+                      "sequential replacement method, the ABC-proxy), median of %d runs of %.2f s"
+                      % (n_raw, n, runs, m_culs),
+            "build": flags or "shipped oracle/liboracle.so (gcc unavailable here)",
+            "pinned_core": core, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "runs_s": [round(x, 3) for x in t_culs],
+            "this_method": {"value": round(n / m_rw / 1e6, 4), "median_s": round(m_rw, 3),
+                            "runs_s": [round(x, 3) for x in t_rw], "what": "orc_rw: the oracle of this package's pass"},
+            "strash_median_s": round(m_st, 3),
+            "e2e_value": round(n_raw / (m_st + m_culs) / 1e6, 4),
+            "qor_ands": {"rw_culs": O.n_ands(rc), "rw": O.n_ands(rp)}}
 
 
 def bench_batch(a):

@@ -49,11 +49,21 @@ class OpInfo(ctypes.Structure):
                 ("candidates", ctypes.c_int64), ("selected", ctypes.c_int64), ("new_nodes", ctypes.c_int64),
                 ("rounds", ctypes.c_int64), ("levels_enumerated", ctypes.c_int64), ("est_gain", ctypes.c_int64),
                 ("kernel_sample_ms", ctypes.c_double), ("kernel_samples", ctypes.c_int64),
-                ("kernel_sample_nodes", ctypes.c_int64)]
+                ("kernel_sample_nodes", ctypes.c_int64), ("hash_probes", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+
+class CecInfo(ctypes.Structure):
+    _fields_ = [("equivalent", ctypes.c_int32), ("exhaustive", ctypes.c_int32), ("po", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("pattern", ctypes.c_int64), ("patterns", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+GLS_SIM_EXHAUSTIVE_PIS = 16
 
 _lib = None
 
@@ -91,6 +101,8 @@ _SIGS = {
     "gls_enumerate_cuts": (ctypes.c_int, [_P, _P]),
     "gls_evaluate": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
     "gls_npn_tables": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gls_cec": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(CecInfo)]),
+    "gls_simulate": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, _P]),
     "gls_gen_random": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_uint64, ctypes.POINTER(_P)]),
     "gls_gen_geometric": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -296,6 +308,20 @@ class Aig:
         best = np.zeros(s.nObjs * 4, np.int32)
         self.ctx.check(lib().gls_evaluate(self._h, 1 if use_zeros else 0, _ptr(best)), "gls_evaluate")
         return best
+
+    def cec(self, other, words=64, seed=1):
+        """Equivalence with `other` by device simulation (exhaustive for <= 16 PIs)."""
+        info = CecInfo()
+        self.ctx.check(lib().gls_cec(self._h, other._h, int(words), int(seed), ctypes.byref(info)), "gls_cec")
+        return info.as_dict()
+
+    def simulate(self, words=1, seed=1, exhaustive=False):
+        """PO values over words*64 patterns: uint64 array [nPOs, words]."""
+        s = self.stats()
+        out = np.zeros((max(s.nPOs, 1), words), np.uint64)
+        self.ctx.check(lib().gls_simulate(self._h, int(words), int(seed), 1 if exhaustive else 0, _ptr(out)),
+                       "gls_simulate")
+        return out[:s.nPOs]
 
     def ps(self):
         """The reference's `ps` line (src/aig_manager.cu:604-612)."""

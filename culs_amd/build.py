@@ -20,7 +20,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Wall", "-Wno-unused-function", "-Wno-unused-result",
           "-I" + SRC, "-I" + os.path.join(HERE, "..", "include")]
 
-LIB_SOURCES = ["strash.hip", "rewrite.hip", "batch.hip", "balance.hip", "capi.cpp", "host_io.cpp",
+LIB_SOURCES = ["strash.hip", "rewrite.hip", "batch.hip", "balance.hip", "sim.hip", "capi.cpp", "host_io.cpp",
                "cpu_strash.cpp", "server.cpp"]
 
 

@@ -589,3 +589,39 @@ extern "C" int gls_gen_multiplier(gls_ctx *ctx, int32_t nbits, double redundant,
     int rc = gen_multiplier_arrays(nbits, redundant, seed, f0, f1, pos);
     return finish_gen(ctx, rc, 2 * nbits, f0, f1, pos, out);
 }
+
+// ---------------------------------------------------------------------------
+// equivalence checking by simulation
+// ---------------------------------------------------------------------------
+// rows of simulation values held at once by gls_cec (both AIGs together)
+static constexpr size_t kCecRowBudget = (size_t)8 << 30;
+
+extern "C" int gls_cec(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gls_cec_info *out) {
+    if (!a || !b || !out) return GLS_ERR_ARG;
+    gls_ctx *ctx = a->ctx;
+    if (b->ctx != ctx) return gls_fail(ctx, GLS_ERR_ARG, "cec: the AIGs belong to different contexts");
+    if (a->nPIs != b->nPIs || a->nPOs != b->nPOs)
+        return gls_fail(ctx, GLS_ERR_ARG, "cec: interfaces differ (%d vs %d PIs, %d vs %d POs)", a->nPIs, b->nPIs,
+                        a->nPOs, b->nPOs);
+    if (a->nPIs > GLS_SIM_EXHAUSTIVE_PIS && nWords < 1)
+        return gls_fail(ctx, GLS_ERR_ARG, "cec: nWords must be positive");
+    GLS_TRY(aig_ensure_device(a));
+    GLS_TRY(aig_ensure_device(b));
+    const size_t bytes = sim_rows_bytes((int64_t)a->nObjs + b->nObjs, a->nPIs, nWords, kCecRowBudget);
+    uint64_t *rows = static_cast<uint64_t *>(ctx->pool.get(bytes));
+    unsigned long long *d_first = pool_get<unsigned long long>(ctx, 1);
+    uint32_t *d_err = pool_get<uint32_t>(ctx, 1);
+    int rc = GLS_OK;
+    if (!rows || !d_first || !d_err) rc = gls_fail(ctx, GLS_ERR_NOMEM, "cec: device allocation of %zu bytes", bytes);
+    if (rc == GLS_OK) rc = dev_sim_check(ctx, a->dev, b->dev, nWords, seed, rows, bytes, d_first, d_err, out);
+    pool_put(ctx, rows);
+    pool_put(ctx, d_first);
+    pool_put(ctx, d_err);
+    return rc;
+}
+
+extern "C" int gls_simulate(gls_aig *a, int32_t nWords, uint64_t seed, int32_t exhaustive, uint64_t *po_words) {
+    if (!a || (a->nPOs > 0 && !po_words) || nWords < 1) return a ? gls_fail(a->ctx, GLS_ERR_ARG, "simulate: bad arguments") : GLS_ERR_ARG;
+    GLS_TRY(aig_ensure_device(a));
+    return dev_simulate(a->ctx, a->dev, nWords, seed, exhaustive ? 1 : 0, po_words);
+}

@@ -76,6 +76,17 @@ __device__ __forceinline__ void sweep_round_end(SweepCtl *ctl, uint32_t k) {
     }
 }
 
+// Adds every thread's x to *dst with one global atomic per workgroup; every
+// thread of the workgroup must call it.
+__device__ __forceinline__ void block_add_u64(unsigned long long *dst, uint32_t x) {
+    __shared__ unsigned long long s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    if (x) atomicAdd(&s_sum, (unsigned long long)x);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(dst, s_sum);
+}
+
 inline int grid_for(int64_t n, int cap = 4096) {
     int64_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;

@@ -135,3 +135,11 @@ int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const 
                  int useZeros, int32_t *d_best);
 void dev_graph_free(gls_ctx *ctx, DevGraph &g);
 int dev_graph_alloc(gls_ctx *ctx, DevGraph &g, int32_t nObjs, int32_t nPIs, int32_t nPOs);
+// simulation (sim.hip).  dev_sim_check compares the POs of a and b with rows
+// (rowsBytes bytes), d_first (one u64) and d_err (one u32) supplied by the
+// caller; dev_simulate copies PO words to the host ([po][nWords]).
+int dev_sim_check(gls_ctx *ctx, const DevGraph &a, const DevGraph &b, int32_t nWords, uint64_t seed, uint64_t *rows,
+                  size_t rowsBytes, unsigned long long *d_first, uint32_t *d_err, gls_cec_info *out);
+int dev_simulate(gls_ctx *ctx, const DevGraph &a, int32_t nWords, uint64_t seed, int32_t exhaustive, uint64_t *h_out);
+// Row bytes dev_sim_check needs for the first batch of words.
+size_t sim_rows_bytes(int64_t objs, int32_t nPIs, int32_t nWords, size_t budget);

@@ -144,10 +144,12 @@ struct CmdMan {
         registerCommand("read", readHandler);
         registerCommand("write", writeHandler);
         registerCommand("ps", psHandler);
+        registerCommand("print_stats", psHandler);
         registerCommand("time", timeHandler);
         registerCommand("st", strashHandler);
         registerCommand("strash", strashHandler);
         registerCommand("rw", rewriteHandler);
+        registerCommand("rewrite", rewriteHandler);
         registerCommand("rwz", rewriteHandler);
         registerCommand("b", balanceHandler);
         registerCommand("balance", balanceHandler);

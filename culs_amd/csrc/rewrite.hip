@@ -1880,7 +1880,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         if (herr) return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr);
         tm.mark(); // 6
         gGuard.free_now();
-        rc = dev_strash(ctx, comb, out, depth, nullptr, tagC, tagged ? tagOut : nullptr, tagged ? tagDepth : nullptr);
+        gls_op_info stInfo = {};
+        rc = dev_strash(ctx, comb, out, depth, &stInfo, tagC, tagged ? tagOut : nullptr, tagged ? tagDepth : nullptr);
         combGuard.free_now();
         tm.mark(); // 7
         GLS_HIP(ctx, hipStreamSynchronize(st));
@@ -1899,6 +1900,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->rounds = rounds;
             info->levels_enumerated = (int64_t)levelStart.size() - 2;
             info->est_gain = (int64_t)estGain;
+            info->hash_probes = stInfo.hash_probes;
             double kms = 0;
             int64_t kunits = 0, klaunches = 0;
             for (int i = 0; i < ctx->ksamp_n; i++) {

@@ -125,8 +125,9 @@ constexpr unsigned long long VAL_MARK = 0xFFFFFFFE00000000ull;
 
 __global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
                             const uint8_t *__restrict__ dead, unsigned long long *ht, uint64_t mask, uint32_t *slotTmp,
-                            uint32_t *err) {
+                            uint32_t *err, unsigned long long *probes) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    uint32_t np = 0;
     for (int64_t v = nPIs + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
         if (dead[v]) continue;
         uint32_t l0 = f0[v], l1 = f1[v];
@@ -139,6 +140,7 @@ __global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict_
             unsigned long long prev = KEY_EMPTY;
             __hip_atomic_compare_exchange_strong(&ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ++np;
             if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
             s = (s + 1) & mask;
         }
@@ -146,6 +148,7 @@ __global__ void k_preinsert(int32_t n, int32_t nPIs, const uint32_t *__restrict_
         atomicMin(&ht[2 * (size_t)slot + 1], (unsigned long long)v);
         slotTmp[v] = slot;
     }
+    block_add_u64(probes, np);
 }
 
 __global__ void k_canon(int32_t n, int32_t nPIs, const uint8_t *__restrict__ dead, unsigned long long *ht,
@@ -177,6 +180,7 @@ struct WaveArgs {
     uint32_t *pending, *order;
     uint32_t *fcnt, *fstart;        // size and start (in order[]) of every wave's frontier
     uint32_t *depth, *err;
+    unsigned long long *probes;     // table slot inspections (key probes and value updates)
 };
 
 __device__ __forceinline__ unsigned long long pack_rl(uint32_t lit, uint32_t lev) {
@@ -220,7 +224,7 @@ __device__ __forceinline__ uint32_t map_fanin(const WaveArgs &a, uint32_t lit, u
 // Returns true for a non-trivial node (slot and level filled, rl not yet).
 template <int SCOPE>
 __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t raw0,
-                                               uint32_t raw1, uint32_t &slotOut, uint32_t &levOut) {
+                                               uint32_t raw1, uint32_t &slotOut, uint32_t &levOut, uint32_t &np) {
     uint32_t la, lb;
     uint32_t l0 = map_fanin<SCOPE>(a, raw0, la), l1 = map_fanin<SCOPE>(a, raw1, lb);
     if (l0 > l1) { uint32_t t = l0; l0 = l1; l1 = t; t = la; la = lb; lb = t; }
@@ -242,6 +246,7 @@ __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, ui
         const uint32_t pre = a.locOf[v];
         if (l0 == r0 && l1 == r1 && pre != NO_SLOT) {
             __hip_atomic_fetch_min(vptr(a, pre), (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
+            ++np;
             slotOut = pre;
             return true;
         }
@@ -252,6 +257,7 @@ __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, ui
         unsigned long long prev = KEY_EMPTY;
         __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
                                              __ATOMIC_RELAXED, SCOPE);
+        ++np;
         if (prev == KEY_EMPTY || prev == key) {
             loc = loc_of_slot<SCOPE>(a, s);
             __hip_atomic_fetch_min(vptr(a, loc), (unsigned long long)val, __ATOMIC_RELAXED, SCOPE);
@@ -267,8 +273,8 @@ __device__ __forceinline__ bool hash_node_lits(const WaveArgs &a, uint32_t k, ui
 
 template <int SCOPE>
 __device__ __forceinline__ bool hash_node(const WaveArgs &a, uint32_t k, uint32_t v, uint32_t &slotOut,
-                                          uint32_t &levOut) {
-    return hash_node_lits<SCOPE>(a, k, v, a.f0[v], a.f1[v], slotOut, levOut);
+                                          uint32_t &levOut, uint32_t &np) {
+    return hash_node_lits<SCOPE>(a, k, v, a.f0[v], a.f1[v], slotOut, levOut, np);
 }
 
 // Releases the fanouts of v whose last pending fanin is v.  Up to four
@@ -309,6 +315,7 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
         a.fstart[k + 1] = nb;
         if (cn > 0) atomicMax(a.depth, k);
     }
+    uint32_t np = 0;
     for (uint32_t i = tid; i < pn + cn; i += stride) {
         if (i < pn) {
             resolve<__HIP_MEMORY_SCOPE_AGENT>(a, a.order[ps + i]);
@@ -316,9 +323,10 @@ __global__ void k_wave_multi(uint32_t k, WaveArgs a) {
         }
         const uint32_t v = a.order[cs + i - pn];
         uint32_t slot = 0, lev = 0;
-        if (hash_node<__HIP_MEMORY_SCOPE_AGENT>(a, k, v, slot, lev)) a.rl[v] = pack_rl(INVALID32, lev);
+        if (hash_node<__HIP_MEMORY_SCOPE_AGENT>(a, k, v, slot, lev, np)) a.rl[v] = pack_rl(INVALID32, lev);
         release<__HIP_MEMORY_SCOPE_AGENT>(a, v, [&](uint32_t w) { a.order[nb + atomicAdd(&a.fcnt[k + 1], 1u)] = w; });
     }
+    block_add_u64(a.probes, np);
 }
 
 // Packed per-node record for the in-workgroup sweep, one 64-byte line:
@@ -592,7 +600,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         return;
     }
     int b = 0;
-    uint32_t last = 0;
+    uint32_t last = 0, np = 0;
     while (true) {
         const int nb = b ^ 1;
         const uint32_t ne = s_ecnt[b], nbase = cs + cn;
@@ -630,6 +638,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
                         __hip_atomic_compare_exchange_strong(&a.ht[2 * s], &prev, (unsigned long long)key,
                                                              __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                        ++np;
                         if (prev == KEY_EMPTY || prev == key) {
                             loc = loc_of_slot<__HIP_MEMORY_SCOPE_WORKGROUP>(a, s);
                             break;
@@ -641,6 +650,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
                 }
                 const unsigned long long prev = __hip_atomic_fetch_min(vptr(a, loc), val, __ATOMIC_RELAXED,
                                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                ++np;
                 if (prev != KEY_EMPTY && (uint32_t)(prev >> 32) < k) {
                     fin = true;                    // an earlier wave holds the slot
                     r = 2u * (uint32_t)prev;
@@ -795,6 +805,7 @@ __global__ void __launch_bounds__(WG1) k_waves_1wg(uint32_t k0, uint32_t thresh,
         *kOut = k;
         if (last) atomicMax(a.depth, last);
     }
+    block_add_u64(a.probes, np);
 }
 
 // wave of every processed node from the per-wave ranges of order[]: wave k
@@ -948,7 +959,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     uint32_t *locOf = t.get<uint32_t>(n), *slotTmp = t.get<uint32_t>(n);
     uint32_t *fcnt = t.get<uint32_t>(n + 4), *fstart = t.get<uint32_t>(n + 4);
     unsigned long long *val = t.get<unsigned long long>(n);
-    uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave
+    uint32_t *d_misc = t.get<uint32_t>(8);   // [0] depth, [1] err, [2] next wave, [4..5] probe count
     NodeRec *recs = t.get<NodeRec>(n);
     NodeOfl *ofls = t.get<NodeOfl>(n);
     SweepCtl *ctl = t.get<SweepCtl>(1);
@@ -1001,7 +1012,8 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
     k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cnt, cursor, adj, adjY);
     k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
-    k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotTmp, &d_misc[1]);
+    unsigned long long *d_probes = reinterpret_cast<unsigned long long *>(&d_misc[4]);
+    k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotTmp, &d_misc[1], d_probes);
     k_canon<<<gN, kBlock, 0, st>>>(n, nPIs, dead, ht, slotTmp, locOf);
     k_node_recs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, locOf, adjOff, adj, adjY, recs, ofls);
     GLS_HIP(ctx, hipGetLastError());
@@ -1009,7 +1021,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     // 3. wave sweep: narrow waves run back to back inside one workgroup, wide
     //    ones as one grid-wide kernel each
     WaveArgs a{in.f0, in.f1, nPIs, rl, locOf, wave, ht, val, cap - 1, adjOff, adj, pending, order,
-               fcnt, fstart, &d_misc[0], &d_misc[1]};
+               fcnt, fstart, &d_misc[0], &d_misc[1], d_probes};
     const uint32_t kThresh = WL_FM;
     uint32_t k = 1, waves = 0, cur = 0;
     for (;;) {
@@ -1036,7 +1048,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
             GLS_HIP(ctx, hipGetLastError());
         }
     }
-    uint32_t misc[2] = {0, 0};
+    uint32_t misc[8] = {};
     GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
     if (misc[1]) return gls_fail(ctx, GLS_ERR_INTERNAL, "strash table overflow");
     const int32_t depth = (int32_t)misc[0];
@@ -1093,6 +1105,9 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, hipGetLastError());
     GLS_HIP(ctx, hipStreamSynchronize(st));
     if (depthOut) *depthOut = depth;
-    if (info) info->levels_enumerated = waves;
+    if (info) {
+        info->levels_enumerated = waves;
+        info->hash_probes += (int64_t)((uint64_t)misc[4] | (uint64_t)misc[5] << 32);
+    }
     return GLS_OK;
 }

@@ -58,6 +58,10 @@ typedef struct {
      * they cover and the nodes those launches processed. */
     double kernel_sample_ms;
     int64_t kernel_samples, kernel_sample_nodes;
+    /* strash table slot inspections of the operation's strash sweeps: key
+     * probes (compare-and-swap attempts on a slot) plus value updates through
+     * a pre-inserted key's stored location */
+    int64_t hash_probes;
 } gls_op_info;
 
 /* ---- context ---------------------------------------------------------- */
@@ -196,6 +200,35 @@ int gls_enumerate_cuts(gls_aig *aig, uint32_t *cuts);
 /* Per-node best replacement of a reordered AIG (EvaluateNode, rewrite.cu:378-447).
  * best: nObjs*4 int32 {cut slot or -1, subgraph index, gain, root level}. */
 int gls_evaluate(gls_aig *aig, int32_t use_zeros, int32_t *best);
+
+/* ---- equivalence checking by simulation -------------------------------- */
+/* No counterpart in the reference's source: CULS ships its GPU
+ * simulation-based CEC only as a binary (README.md:102-111).  Device
+ * bit-parallel simulation of both AIGs over the same input patterns,
+ * comparing PO i of a with PO i of b (same nPIs and nPOs).  With
+ * nPIs <= GLS_SIM_EXHAUSTIVE_PIS every one of the 2^nPIs patterns is
+ * simulated and `equivalent` is a proof; otherwise nWords * 64 seeded
+ * patterns are, and only a difference is conclusive.
+ * Patterns: exhaustive pattern p assigns PI j (0-based) bit j of p; random
+ * word w of PI j is splitmix64's output function of
+ * seed + ((j << 32 | w) + 1) * 0x9E3779B97F4A7C15.
+ * po / pattern: the smallest differing PO index, and the smallest pattern on
+ * which it differs (-1 when equivalent).  Both handles must belong to one
+ * context. */
+#define GLS_SIM_EXHAUSTIVE_PIS 16
+typedef struct {
+    int32_t equivalent;  /* 1: all POs agree on every simulated pattern */
+    int32_t exhaustive;  /* 1: all 2^nPIs patterns were simulated */
+    int32_t po;          /* smallest differing PO, -1 if none */
+    int32_t pad;
+    int64_t pattern;     /* smallest pattern on which `po` differs, -1 if none */
+    int64_t patterns;    /* patterns simulated */
+} gls_cec_info;
+int gls_cec(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gls_cec_info *out);
+/* PO values of one AIG over nWords * 64 patterns (exhaustive = 1: pattern p
+ * assigns PI j bit j of p, for any nPIs; else the seeded random words above):
+ * po_words[po * nWords + w], bit b of word w = pattern 64 w + b. */
+int gls_simulate(gls_aig *aig, int32_t nWords, uint64_t seed, int32_t exhaustive, uint64_t *po_words);
 
 /* ---- NPN tables and library (host) ------------------------------------ */
 /* Closed-form NPN tables (the content read by rw::CPUSolver::ReadLibrary,

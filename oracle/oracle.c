@@ -1554,6 +1554,38 @@ static void sim_words(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, c
     }
 }
 
+/* Pattern words of the library's simulation (gls_simulate / gls_cec): the
+ * exhaustive pattern p = 64w + b gives PI j bit j of p; the random word w of
+ * PI j is splitmix64's output function at state seed + (j << 32 | w) + 1
+ * golden-gamma steps.  An own definition (the reference's simulation-based
+ * CEC is closed-source, README.md:102-111). */
+static uint64_t sim_pi_word(int j, uint64_t w, uint64_t seed, int exhaustive) {
+    if (!exhaustive) {
+        uint64_t st = seed + ((((uint64_t)j << 32) | w)) * 0x9E3779B97F4A7C15ULL;
+        return splitmix(&st);
+    }
+    uint64_t x = 0;
+    for (int b = 0; b < 64; b++) {
+        uint64_t p = w * 64 + (uint64_t)b;
+        if (j < 64 && ((p >> j) & 1)) x |= 1ull << b;
+    }
+    return x;
+}
+
+void orc_sim_po_words(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                      int nWords, uint64_t seed, int exhaustive, uint64_t *out) {
+    uint64_t *val = malloc(sizeof(uint64_t) * (size_t)(nObjs > 0 ? nObjs : 1));
+    uint64_t *pi = malloc(sizeof(uint64_t) * (size_t)(nPIs + 1));
+    for (int w = 0; w < nWords; w++) {
+        for (int j = 0; j < nPIs; j++) pi[j] = sim_pi_word(j, (uint64_t)w, seed, exhaustive);
+        sim_words(nObjs, nPIs, f0, f1, pi, val);
+        for (int o = 0; o < nPOs; o++)
+            out[(size_t)o * nWords + w] = val[ID(pos[o])] ^ (CMP(pos[o]) ? ~0ull : 0);
+    }
+    free(val);
+    free(pi);
+}
+
 int orc_sim_cmp(int nA, const uint32_t *a0, const uint32_t *a1, const uint32_t *aPos,
                 int nB, const uint32_t *b0, const uint32_t *b1, const uint32_t *bPos,
                 int nPIs, int nPOs, int nWords, uint64_t seed) {

@@ -84,6 +84,12 @@ int orc_sim_cmp(int nObjsA, const uint32_t *a0, const uint32_t *a1, const uint32
                 int nObjsB, const uint32_t *b0, const uint32_t *b1, const uint32_t *bPos,
                 int nPIs, int nPOs, int nWords, uint64_t seed);
 
+/* PO words over nWords*64 patterns with the pattern definition of
+ * gls_simulate (exhaustive: pattern p gives PI j bit j of p; else splitmix64
+ * words keyed by (j << 32 | w) under seed): out[po * nWords + w]. */
+void orc_sim_po_words(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                      int nWords, uint64_t seed, int exhaustive, uint64_t *out);
+
 /* MFFC size of root within the given leaves, as counted by the evaluation. */
 int orc_mffc(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
              int root, const int *leaves, int nL, int *tabSize);

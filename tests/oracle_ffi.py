@@ -55,6 +55,8 @@ def _load(path):
     L.orc_rw_culs.argtypes = [i32, i32, _P, _P, i32, _P, i32, i32, _P, _P, _P, _P]
     L.orc_sim_cmp.argtypes = [i32, _P, _P, _P, i32, _P, _P, _P, i32, i32, i32, u64]
     L.orc_depth.argtypes = [i32, i32, _P, _P]
+    L.orc_sim_po_words.argtypes = [i32, i32, _P, _P, i32, _P, i32, u64, i32, _P]
+    L.orc_sim_po_words.restype = None
     return L
 
 
@@ -173,6 +175,15 @@ def sim_cmp(a, b, words=64, seed=1):
     assert nPIs == nPIs2 and len(ap) == len(bp)
     a0, a1, ap, b0, b1, bp = map(u32, (a0, a1, ap, b0, b1, bp))
     return lib().orc_sim_cmp(len(a0), P(a0), P(a1), P(ap), len(b0), P(b0), P(b1), P(bp), nPIs, len(ap), words, seed)
+
+
+def sim_po_words(g, words, seed=1, exhaustive=False):
+    """PO values [nPOs, words] under the pattern definition of gls_simulate."""
+    nPIs, f0, f1, pos = g
+    f0, f1, pos = u32(f0), u32(f1), u32(pos)
+    out = np.zeros((max(len(pos), 1), words), np.uint64)
+    lib().orc_sim_po_words(len(f0), nPIs, P(f0), P(f1), len(pos), P(pos), words, seed, 1 if exhaustive else 0, P(out))
+    return out[:len(pos)]
 
 
 def depth(nPIs, f0, f1):
