@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) 5 times, to scale HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="rw10m: skip the batched and served legs that the default run appends")
     ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline runs (median)")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
@@ -125,9 +127,11 @@ def main():
     import culs_amd
 
     if a.workload == "batch":
-        return bench_batch(a)
+        print(json.dumps(bench_batch(a)), flush=True)
+        return
     if a.workload == "serve":
-        return bench_serve(a)
+        print(json.dumps(bench_serve(a)), flush=True)
+        return
     ctx = culs_amd.Context(0)
     if not ctx.has_device:
         raise SystemExit("no GPU")
@@ -286,6 +290,19 @@ def main():
         res["e2e"]["vs_cpu"] = round(res["e2e"]["value"] / cpu["e2e_value"], 3)
     if cpu is not None:
         res["vs_cpu"] = round(value / cpu["value"], 3)
+    if not a.no_batch:
+        # the batched and served workloads, in the same run (their own keys;
+        # `value` stays the single-AIG rw throughput)
+        for x in (out, src, raw):
+            x.free()
+        b = bench_batch(a)
+        res["batch"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "loop", "speedup_vs_loop",
+                                          "identical_to_alone", "config", "stages_ms")}
+        log("batch: %s" % res["batch"])
+        v = bench_serve(a)
+        res["serve"] = {k: v[k] for k in ("metric", "value", "unit", "served", "loop", "throughput_vs_loop",
+                                          "capacity", "identical_to_loop", "config")}
+        log("serve: %s" % res["serve"])
     print(json.dumps(res), flush=True)
 
 
@@ -414,12 +431,10 @@ def bench_batch(a):
     t_batch = sum(times) / len(times)
     info = out[0].last_op()
     if a.no_loop:
-        print(json.dumps({"metric": "batched rw Mnodes/s, %d mixed AIGs" % len(aigs),
-                          "value": round(n_in / t_batch / 1e6, 4), "unit": "Mnodes/s",
-                          "ms_per_step": round(t_batch * 1e3, 3), "loop": None,
-                          "stages_ms": {k: round(info[k], 3) for k in ("cuts_ms", "eval_ms", "strash_ms",
-                                                                        "total_ms")}}), flush=True)
-        return
+        return {"metric": "batched rw Mnodes/s, %d mixed AIGs" % len(aigs),
+                "value": round(n_in / t_batch / 1e6, 4), "unit": "Mnodes/s",
+                "ms_per_step": round(t_batch * 1e3, 3), "loop": None,
+                "stages_ms": {k: round(info[k], 3) for k in ("cuts_ms", "eval_ms", "strash_ms", "total_ms")}}
     # the plain loop: the same pass, one AIG at a time (each already on the device)
     c = copies()
     t = time.perf_counter()
@@ -450,7 +465,9 @@ def bench_batch(a):
         "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
                                                       "build_ms", "strash_ms", "total_ms")},
     }
-    print(json.dumps(res), flush=True)
+    for x in out + c + aigs:
+        x.free()
+    return res
 
 
 def bench_serve(a):
@@ -562,7 +579,7 @@ def bench_serve(a):
                      "loop_jobs_per_s": round(len(jobs) / busy, 3), "ratio": round(busy / sat_span, 3)},
         "identical_to_loop": bool(identical),
     }
-    print(json.dumps(res), flush=True)
+    return res
 
 
 if __name__ == "__main__":

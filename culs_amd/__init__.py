@@ -21,6 +21,7 @@ GLS_RW_DATAFLOW_CUTS = 4
 GLS_RW_XCD_CUTS = 8
 GLS_RW_LEVEL_CUTS = 16
 GLS_RW_PAIR_CUTS = 32
+GLS_RW_FULL_STRASH = 64
 
 
 class GlsError(RuntimeError):
@@ -49,7 +50,9 @@ class OpInfo(ctypes.Structure):
                 ("candidates", ctypes.c_int64), ("selected", ctypes.c_int64), ("new_nodes", ctypes.c_int64),
                 ("rounds", ctypes.c_int64), ("levels_enumerated", ctypes.c_int64), ("est_gain", ctypes.c_int64),
                 ("kernel_sample_ms", ctypes.c_double), ("kernel_samples", ctypes.c_int64),
-                ("kernel_sample_nodes", ctypes.c_int64), ("hash_probes", ctypes.c_int64)]
+                ("kernel_sample_nodes", ctypes.c_int64), ("hash_probes", ctypes.c_int64),
+                ("cut_schedule", ctypes.c_int32), ("final_strash", ctypes.c_int32),
+                ("strash_rounds", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -271,13 +274,14 @@ class Aig:
         self.ctx.check(lib().gls_strash(self._h, GLS_ST_CPU if cpu else 0), "gls_strash")
         return self
 
-    def rewrite(self, use_zeros=False, cut_mode=0):
+    def rewrite(self, use_zeros=False, cut_mode=0, full_strash=False):
         """One rw pass (rw -z / rwz with use_zeros).  cut_mode schedules cut
         enumeration: 0 one launch per three levels, GLS_RW_PAIR_CUTS per two,
         GLS_RW_LEVEL_CUTS per level, GLS_RW_COOP_CUTS one cooperative launch,
         GLS_RW_DATAFLOW_CUTS fanin-driven, GLS_RW_XCD_CUTS one persistent
-        kernel on one XCD; the results are identical."""
-        flags = cut_mode
+        kernel on one XCD; full_strash finishes with the wave-sweep strash
+        instead of the incremental one.  The results are identical."""
+        flags = cut_mode | (GLS_RW_FULL_STRASH if full_strash else 0)
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
         return self
 

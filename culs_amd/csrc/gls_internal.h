@@ -118,6 +118,20 @@ int aiger_write(const char *path, int32_t nPIs, const std::vector<uint32_t> &f0,
 // tag's nodes alone would report, i.e. the largest wave among its live nodes.
 int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, gls_op_info *info,
                const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr, uint32_t *tagDepth = nullptr);
+// What a rewrite started from, for the strash of the graph it composed
+// (dev_strash_rw): its input graph g (ids 0..n-1, fanins g0/g1, levels
+// glevel, structural table tOff/tEnt: bucket x holds the ANDs whose fanin of
+// larger id is x, as {first fanin literal, node | second literal's complement
+// << 31}).
+struct RwDelta {
+    int32_t n;
+    const uint32_t *g0, *g1, *glevel;
+    const uint32_t *tOff;
+    const uint2 *tEnt;
+};
+int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &out, int32_t *depth,
+                  gls_op_info *info, const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr,
+                  uint32_t *tagDepth = nullptr);
 int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int32_t> &levelStart,
                 const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr);
 int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGraph &out, int32_t *depth,
@@ -130,7 +144,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn = nullptr,
                 uint32_t **tagOut = nullptr);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
-                       uint32_t *d_cuts, int32_t *d_nref, int mode, const uint8_t *sigBits = nullptr);
+                       uint32_t *d_cuts, int32_t *d_nref, int mode, const uint8_t *sigBits = nullptr,
+                       int *ranMode = nullptr);
 int dev_evaluate(gls_ctx *ctx, const DevGraph &g, const uint32_t *d_cuts, const int32_t *d_nref,
                  int useZeros, int32_t *d_best);
 void dev_graph_free(gls_ctx *ctx, DevGraph &g);

@@ -1502,7 +1502,10 @@ static const bool kSampleLaunches = [] {
 }();
 
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
-                       int32_t *d_nref, int mode, const uint8_t *sigBits) {
+                       int32_t *d_nref, int mode, const uint8_t *sigBits, int *ranMode) {
+    int ranDummy = 0;
+    int &ran = ranMode ? *ranMode : ranDummy;
+    ran = 0;
     hipStream_t st = ctx->stream;
     const int32_t n = g.nObjs;
     ctx->ksamp_n = 0;
@@ -1544,6 +1547,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         if (hctr[0] < (uint32_t)(n - g.nPIs - 1))
             return gls_fail(ctx, GLS_ERR_INTERNAL, "cut enumeration: %u of %d nodes processed", hctr[0],
                             n - g.nPIs - 1);
+        ran = GLS_RW_DATAFLOW_CUTS;
         return GLS_OK;
     }
     if (mode == GLS_RW_XCD_CUTS && L > 16) {
@@ -1580,7 +1584,10 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         pool_put(ctx, d_starts);
         pool_put(ctx, ctl);
         if (rc != GLS_OK) return rc;
-        if (hctl[2] == 0) return GLS_OK;
+        if (hctl[2] == 0) {
+            ran = GLS_RW_XCD_CUTS;
+            return GLS_OK;
+        }
         // not all on one XCD, or a wait overran: redo with one launch per level
         ctx->ksamp_n = 0;
         ctx->ksamp_units.clear();
@@ -1615,6 +1622,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // the pool block is reused only by later work on the same stream
         pool_put(ctx, d_starts);
         if (e != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cooperative cut launch: %s", hipGetErrorString(e));
+        ran = GLS_RW_COOP_CUTS;
     } else {
         // default: three levels per launch (k_cuts_triple); GLS_RW_PAIR_CUTS:
         // two (k_cuts_pair); GLS_RW_LEVEL_CUTS: one (k_cuts_level).  A group
@@ -1622,6 +1630,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // recomputed cones would only add work there, so its levels get one
         // launch each.
         const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : 3;
+        ran = step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS : 0;
         struct CutLaunch { int kind; int32_t s, m, m2, e; };
         std::vector<CutLaunch> plan;
         plan.reserve((size_t)L / step + 1);
@@ -1775,10 +1784,11 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             if (!sigBits) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: signature bits");
             GLS_TRY(sig_bits_for_tags(ctx, n, tagR, sigBits));
         }
+        int cutRan = 0;
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
                                    flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS |
                                             GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS),
-                                   sigBits));
+                                   sigBits, &cutRan));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr};
         GLS_TRY(build_table(ctx, t, g, gv));
@@ -1868,6 +1878,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             if (!tagC) return gls_fail(ctx, GLS_ERR_NOMEM, "rewrite: tags");
             GLS_HIP(ctx, hipMemcpyAsync(tagC, tagR, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
         }
+        const bool fullStrash = (flags & GLS_RW_FULL_STRASH) != 0;
         if (nCand > 0 && totalNew + nSel > 0)
             k_build<<<blocks(nCand, EVAL_BLOCK), EVAL_BLOCK, 0, st>>>(n, (int32_t)nCand, candIds, gv, L, cuts, best,
                                                                        state, newOff, comb.f0, comb.f1, repl, d_err,
@@ -1879,9 +1890,19 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         GLS_HIP(ctx, hipStreamSynchronize(st));
         if (herr) return gls_fail(ctx, GLS_ERR_INTERNAL, "rewrite: inconsistent plan (%u)", herr);
         tm.mark(); // 6
-        gGuard.free_now();
         gls_op_info stInfo = {};
-        rc = dev_strash(ctx, comb, out, depth, &stInfo, tagC, tagged ? tagOut : nullptr, tagged ? tagDepth : nullptr);
+        if (fullStrash) {
+            gGuard.free_now();
+            rc = dev_strash(ctx, comb, out, depth, &stInfo, tagC, tagged ? tagOut : nullptr,
+                            tagged ? tagDepth : nullptr);
+        } else {
+            // the incremental strash reads the input graph's fanins: g is
+            // released only after it
+            const RwDelta delta{n, g.f0, g.f1, g.level, gv.tOff, gv.tEnt};
+            rc = dev_strash_rw(ctx, comb, delta, out, depth, &stInfo, tagC, tagged ? tagOut : nullptr,
+                               tagged ? tagDepth : nullptr);
+            gGuard.free_now();
+        }
         combGuard.free_now();
         tm.mark(); // 7
         GLS_HIP(ctx, hipStreamSynchronize(st));
@@ -1901,6 +1922,9 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->levels_enumerated = (int64_t)levelStart.size() - 2;
             info->est_gain = (int64_t)estGain;
             info->hash_probes = stInfo.hash_probes;
+            info->final_strash = stInfo.final_strash;
+            info->cut_schedule = cutRan;
+            info->strash_rounds = stInfo.strash_rounds;
             double kms = 0;
             int64_t kunits = 0, klaunches = 0;
             for (int i = 0; i < ctx->ksamp_n; i++) {

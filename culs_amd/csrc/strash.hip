@@ -891,6 +891,329 @@ __global__ void k_map_pos(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict
     for (int64_t i = tid; i < nPOs; i += stride) opos[i] = final_lit(pos[i], nPIs, rl, newId);
 }
 
+// ---------------------------------------------------------------------------
+// Strash of a graph composed by a rewrite (dev_strash_rw)
+// ---------------------------------------------------------------------------
+// Longest-path levels of a topologically ordered position array, by one
+// workgroup: lev[p] = 0 where fa[p] == NOFAN, else 1 + max(lev[fa[p]],
+// lev[fb[p]]), fanin positions below p.  Positions go in chunks of SW_WG, one
+// per thread: fanins below the chunk are read from lev (written by this
+// workgroup), the others are waited for in LDS, one barrier per round.  Each
+// round completes at least the smallest pending position, so a chunk needs at
+// most SW_WG rounds; a fanin at or above its position sets *err instead.
+constexpr int SW_WG = 1024;
+constexpr uint32_t NOFAN = 0xFFFFFFFFu;
+// rounds of the incremental strash before it gives way to the wave sweep
+constexpr uint32_t kRwRoundLimit = 4096;
+
+__global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint32_t *__restrict__ fa,
+                                                      const uint32_t *__restrict__ fb, uint32_t *lev,
+                                                      uint32_t *err, uint32_t *maxLev) {
+    // two buffers: chunk c uses buffer c & 1 while each thread marks its entry
+    // of the other one unknown for chunk c + 1.  An entry is one word, the
+    // level or NOLEV, so a reader sees either nothing or the final value.
+    constexpr uint32_t NOLEV = 0xFFFFFFFFu;
+    __shared__ uint32_t s_lev[2][SW_WG];
+    __shared__ uint32_t s_max;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_max = 0;
+    s_lev[0][tid] = NOLEV;
+    uint32_t na = tid < nPos ? fa[tid] : NOFAN, nb = tid < nPos ? fb[tid] : NOFAN;
+    uint32_t mx = 0;
+    bool anyBad = false;
+    __syncthreads();
+    for (uint32_t c = 0, h = 0; c < nPos; c += SW_WG, h ^= 1u) {
+        s_lev[h ^ 1u][tid] = NOLEV;
+        const uint32_t p = c + tid;
+        const bool active = p < nPos;
+        const uint32_t a = na, b = nb;
+        const uint32_t q = p + SW_WG;          // the next chunk's fanins are in flight meanwhile
+        na = q < nPos ? fa[q] : NOFAN;
+        nb = q < nPos ? fb[q] : NOFAN;
+        const bool leaf = !active || a == NOFAN;
+        const bool bad = !leaf && (a >= p || b >= p);
+        bool needA = !leaf && !bad && a >= c, needB = !leaf && !bad && b >= c;
+        uint32_t la = 0, lb = 0;
+        if (!leaf && !bad && !needA) la = lev[a];
+        if (!leaf && !bad && !needB) lb = lev[b];
+        bool done = leaf || bad;
+        uint32_t val = 0;
+        if (active && leaf) s_lev[h][tid] = 0;
+        // one barrier per round: a value seen in this round or the next is final
+        for (uint32_t round = 0;; ++round) {
+            if (!done) {
+                if (needA) {
+                    const uint32_t x = s_lev[h][a - c];
+                    if (x != NOLEV) { la = x; needA = false; }
+                }
+                if (needB) {
+                    const uint32_t x = s_lev[h][b - c];
+                    if (x != NOLEV) { lb = x; needB = false; }
+                }
+                if (!needA && !needB) {
+                    val = 1u + (la > lb ? la : lb);
+                    s_lev[h][tid] = val;
+                    done = true;
+                }
+            }
+            if (!__syncthreads_or(!done)) break;
+            if (round >= SW_WG) { anyBad = true; break; }   // unreachable for fanins below their position
+        }
+        anyBad |= bad;
+        if (active) lev[p] = val;
+        mx = val > mx ? val : mx;
+        if (__syncthreads_or(anyBad)) break;
+    }
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    if (tid == 0) {
+        atomicMax(maxLev, s_max);
+        if (anyBad) atomicOr(err, 1u);
+    }
+}
+
+// A topological key of the composed graph: starting from the input graph's
+// levels (new nodes from 0), every live AND is raised to one above its fanins'
+// keys until nothing moves.  Keys only grow and are bounded, so the loop
+// ends; the fixpoint orders every edge (key of a fanin < key of its fanout).
+// Most old nodes keep their old level; only nodes above a replaced root whose
+// replacement sits higher, and the new nodes, move, a few steps each.
+__global__ void k_rw_key_init(int32_t n, int32_t N, const uint32_t *__restrict__ glevel, uint32_t *key) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) key[v] = v < n ? glevel[v] : 0u;
+}
+
+__global__ void k_rw_key_relax(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
+                               const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead, uint32_t *key,
+                               uint32_t *moved) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    uint32_t mv = 0;
+    for (int64_t v = nPIs + 1 + tid; v < N; v += stride) {
+        if (dead[v]) continue;
+        const uint32_t ka = __hip_atomic_load(&key[LID(f0[v])], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t kb = __hip_atomic_load(&key[LID(f1[v])], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t need = 1u + (ka > kb ? ka : kb);
+        if (__hip_atomic_load(&key[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            atomicMax(&key[v], need);
+            ++mv;
+        }
+    }
+    if (mv) atomicAdd(moved, mv);
+}
+
+__global__ void k_iota_u32(int32_t N, uint32_t *x) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) x[v] = (uint32_t)v;
+}
+
+// position of every object in the key order
+__global__ void k_rw_pos(int32_t N, const uint32_t *__restrict__ order, uint32_t *posOf) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = tid; p < N; p += stride) posOf[order[p]] = (uint32_t)p;
+}
+
+// Fanin positions of every live AND at its sweep position; PIs, the constant
+// and dangling nodes have none.
+__global__ void k_rw_fanpos(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                            const uint8_t *__restrict__ dead, const uint32_t *__restrict__ posOf, uint32_t *fa,
+                            uint32_t *fb, uint32_t *err) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) {
+        const uint32_t p = posOf[v];
+        if (p >= (uint32_t)N) { atomicOr(err, 8u); continue; }   // not a permutation: no write out of range
+        if (v <= nPIs || dead[v]) { fa[p] = NOFAN; fb[p] = NOFAN; continue; }
+        fa[p] = posOf[LID(f0[v])];
+        fb[p] = posOf[LID(f1[v])];
+    }
+}
+
+// Wave of every object (0 for PIs, the constant and dangling nodes), the
+// initial representative literal (itself), and the first dirty set: live new
+// nodes and live old nodes with a redirected fanin.
+__global__ void k_rw_seed(int32_t n, int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
+                          const uint32_t *__restrict__ f1, const uint32_t *__restrict__ g0,
+                          const uint32_t *__restrict__ g1, const uint8_t *__restrict__ dead,
+                          const uint32_t *__restrict__ posOf, const uint32_t *__restrict__ levPos, uint32_t *wave,
+                          uint32_t *rep, uint32_t *inD, uint32_t *dlist, uint32_t *dcount) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) {
+        const bool live = v > nPIs && !dead[v];
+        wave[v] = live ? levPos[posOf[v]] : 0u;
+        rep[v] = 2u * (uint32_t)v;
+        const bool d = live && (v >= n || f0[v] != g0[v] || f1[v] != g1[v]);
+        inD[v] = d ? 1u : 0u;
+        if (d) dlist[atomicAdd(dcount, 1u)] = (uint32_t)v;
+    }
+}
+
+__device__ __forceinline__ uint32_t rep_lit(const uint32_t *rep, int32_t nPIs, uint32_t lit) {
+    const uint32_t u = LID(lit);
+    return (u <= (uint32_t)nPIs ? 2u * u : rep[u]) ^ LC(lit);
+}
+
+// Round, step 1: every dirty node's key over the representatives of the
+// previous round; trivial ANDs resolve at once (hash_table.h:435-449 rules),
+// the others enter this round's table with value (wave << 32 | id), kept by
+// minimum.  dkey keeps the key, dslot its slot (NO_SLOT: resolved, nrep).
+__global__ void k_rw_insert(uint32_t nD, const uint32_t *__restrict__ dlist, int32_t nPIs,
+                            const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                            const uint32_t *__restrict__ rep, const uint32_t *__restrict__ wave,
+                            unsigned long long *ht, uint64_t mask, uint2 *dkey, uint32_t *dslot, uint32_t *nrep,
+                            uint32_t *err, unsigned long long *probes) {
+    uint32_t np = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nD; i += stride) {
+        const uint32_t v = dlist[i];
+        uint32_t l0 = rep_lit(rep, nPIs, f0[v]), l1 = rep_lit(rep, nPIs, f1[v]);
+        if (l0 > l1) { const uint32_t t = l0; l0 = l1; l1 = t; }
+        uint32_t r = INVALID32;
+        if (l0 == l1) r = l0;
+        else if (l0 == (l1 ^ 1u)) r = 0;
+        else if (l0 == 1u) r = l1;
+        else if (l0 == 0u) r = 0;
+        dkey[i] = make_uint2(l0, l1);
+        if (r != INVALID32) { nrep[i] = r; dslot[i] = NO_SLOT; continue; }
+        const uint64_t key = and_key(l0, l1);
+        uint64_t s = mix64(key) & mask;
+        uint32_t slot = NO_SLOT;
+        for (uint64_t probe = 0; probe <= mask; ++probe) {
+            unsigned long long prev = KEY_EMPTY;
+            __hip_atomic_compare_exchange_strong(&ht[2 * s], &prev, (unsigned long long)key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ++np;
+            if (prev == KEY_EMPTY || prev == key) { slot = (uint32_t)s; break; }
+            s = (s + 1) & mask;
+        }
+        nrep[i] = INVALID32;
+        dslot[i] = slot;
+        if (slot == NO_SLOT) { atomicOr(err, 2u); continue; }
+        atomicMin(&ht[2 * (size_t)slot + 1], ((unsigned long long)wave[v] << 32) | v);
+        ++np;
+    }
+    block_add_u64(probes, np);
+}
+
+// The node of the rewrite's input graph with this key, if any: a scan of its
+// structural table bucket of the fanin of larger id (entries {first fanin
+// literal, node | complement of the second << 31}).  Keys over a new node
+// have none.
+__device__ __forceinline__ int32_t old_holder(int32_t n, const uint32_t *__restrict__ tOff,
+                                              const uint2 *__restrict__ tEnt, uint32_t l0, uint32_t l1,
+                                              uint32_t &np) {
+    if (LID(l1) >= (uint32_t)n || LID(l0) >= (uint32_t)n) return -1;
+    const uint32_t hc = LC(l1) << 31, e = tOff[LID(l1) + 1];
+    for (uint32_t k = tOff[LID(l1)]; k < e; k++) {
+        const uint2 x = tEnt[k];
+        ++np;
+        if (x.x == l0 && (x.y & 0x80000000u) == hc) return (int32_t)(x.y & 0x7FFFFFFFu);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void join_dirty(uint32_t *inD, uint32_t v, uint32_t *dlist, uint32_t *dcount,
+                                           uint32_t dcap, uint32_t *err) {
+    if (atomicCAS(&inD[v], 0u, 1u) != 0u) return;
+    const uint32_t k = atomicAdd(dcount, 1u);
+    if (k < dcap) dlist[k] = v;
+    else atomicOr(err, 4u);
+}
+
+// Round, step 2: a non-trivial dirty node takes the smallest (wave, id) among
+// this round's dirty nodes of its key and the key's clean holder (a live old
+// node outside the dirty set, which still has its input fanins); a clean
+// holder that loses joins the dirty set.  changed counts representatives that
+// changed.
+__global__ void k_rw_resolve(uint32_t nD, const uint32_t *__restrict__ dlistIn, int32_t n,
+                             const uint8_t *__restrict__ dead, const uint32_t *__restrict__ wave, uint32_t *rep,
+                             uint32_t *inD, const unsigned long long *__restrict__ ht, const uint2 *__restrict__ dkey,
+                             const uint32_t *__restrict__ dslot, const uint32_t *__restrict__ nrep,
+                             const uint32_t *__restrict__ tOff, const uint2 *__restrict__ tEnt, uint32_t *dlist,
+                             uint32_t *dcount, uint32_t dcap, uint32_t *changed, uint32_t *err,
+                             unsigned long long *probes) {
+    uint32_t np = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nD; i += stride) {
+        const uint32_t v = dlistIn[i];
+        uint32_t r = nrep[i];
+        if (r == INVALID32) {
+            const uint32_t slot = dslot[i];
+            if (slot == NO_SLOT) continue;   // table overflow, reported
+            const unsigned long long dmin = ht[2 * (size_t)slot + 1];
+            ++np;
+            unsigned long long win = dmin;
+            const uint2 k = dkey[i];
+            const int32_t c = old_holder(n, tOff, tEnt, k.x, k.y, np);
+            if (c >= 0 && !dead[c] && !__hip_atomic_load(&inD[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                const unsigned long long cv = ((unsigned long long)wave[c] << 32) | (uint32_t)c;
+                if (cv < dmin) win = cv;
+                else join_dirty(inD, (uint32_t)c, dlist, dcount, dcap, err);
+            }
+            r = 2u * (uint32_t)(win & 0xFFFFFFFFull);
+        }
+        if (r != rep[v]) {
+            rep[v] = r;
+            atomicAdd(changed, 1u);
+        }
+    }
+    block_add_u64(probes, np);
+}
+
+// Round, step 3: a live node outside the dirty set whose fanin's
+// representative is no longer the fanin itself joins the dirty set.
+__global__ void k_rw_pull(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
+                          const uint8_t *__restrict__ dead, const uint32_t *__restrict__ rep, uint32_t *inD,
+                          uint32_t *dlist, uint32_t *dcount, uint32_t dcap, uint32_t *err) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = nPIs + 1 + tid; v < N; v += stride) {
+        if (dead[v] || inD[v]) continue;
+        const uint32_t a = LID(f0[v]), b = LID(f1[v]);
+        const bool moved = (a > (uint32_t)nPIs && rep[a] != 2u * a) || (b > (uint32_t)nPIs && rep[b] != 2u * b);
+        if (moved) join_dirty(inD, (uint32_t)v, dlist, dcount, dcap, err);
+    }
+}
+
+__global__ void k_rw_survivors(int32_t N, int32_t nPIs, const uint8_t *__restrict__ dead,
+                               const uint32_t *__restrict__ rep, uint8_t *surv) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) surv[v] = v > nPIs && !dead[v] && rep[v] == 2u * (uint32_t)v;
+}
+
+__device__ __forceinline__ uint32_t out_lit(uint32_t lit, int32_t nPIs, const uint32_t *rep, const uint32_t *newId) {
+    const uint32_t r = rep_lit(rep, nPIs, lit);
+    return 2u * newId[LID(r)] | LC(r);
+}
+
+// Output fanins and, for the level sweep, their positions (= output ids).
+__global__ void k_rw_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ f0,
+                          const uint32_t *__restrict__ f1, const uint32_t *__restrict__ rep,
+                          const uint32_t *__restrict__ newId, uint32_t *of0, uint32_t *of1, uint32_t *fa,
+                          uint32_t *fb, const uint32_t *__restrict__ tagIn, uint32_t *tagOut) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i <= nPIs; i += stride) { fa[i] = NOFAN; fb[i] = NOFAN; }
+    for (int64_t p = tid; p < m; p += stride) {
+        const uint32_t v = sorted[p];
+        uint32_t a = out_lit(f0[v], nPIs, rep, newId), b = out_lit(f1[v], nPIs, rep, newId);
+        if (a > b) { const uint32_t t = a; a = b; b = t; }
+        const int64_t o = nPIs + 1 + p;
+        of0[o] = a;
+        of1[o] = b;
+        fa[o] = LID(a);
+        fb[o] = LID(b);
+        if (tagOut) tagOut[o] = tagIn[v];
+    }
+}
+
+__global__ void k_rw_pos_out(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict__ pos,
+                             const uint32_t *__restrict__ rep, const uint32_t *__restrict__ newId, uint32_t *opos) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i < nPOs; i += stride) opos[i] = out_lit(pos[i], nPIs, rep, newId);
+}
+
+__global__ void k_fill_u64(size_t n, unsigned long long *p, unsigned long long v) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
 struct Tmp {
     gls_ctx *ctx;
     std::vector<void *> ptrs;
@@ -1108,6 +1431,210 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     if (info) {
         info->levels_enumerated = waves;
         info->hash_probes += (int64_t)((uint64_t)misc[4] | (uint64_t)misc[5] << 32);
+    }
+    return GLS_OK;
+}
+
+// Strash of the graph a rewrite composed (comb: the rewrite's input graph g,
+// ids 0..n-1, with the selected roots' fanouts redirected, plus the new nodes
+// n..N-1).  The result equals dev_strash(comb) array for array; it is computed
+// from what changed instead of a wave-by-wave sweep:
+//  1. dangling nodes, as in dev_strash;
+//  2. waves (levels over the live nodes) by one chunked sweep over a
+//     topological order of comb, sorted by a key relaxed from g's levels;
+//  3. representatives: g is strashed, so a live old node whose fanins are
+//     unchanged and are their own representatives keeps its key, which no
+//     other such node has.  Only the other nodes (new nodes, redirected old
+//     nodes, and what their merges reach) are hashed, in rounds: each round
+//     hashes the dirty set over the previous round's representatives into a
+//     fresh table and compares each key with its clean holder (found in g's
+//     structural table); a representative that changes pulls its fanouts into
+//     the dirty set, a clean holder that loses on (wave, id) joins it.  The
+//     rounds stop when nothing changes: then every node's representative is
+//     the smallest (wave, id) over the nodes of its key with fanins over
+//     representatives, which is the fixpoint the wave sweep reaches (it is
+//     unique: by induction over waves, each wave's keys and minima follow
+//     from the waves before);
+//  4. survivors in (wave, id) order, fanins over representatives, and the
+//     output levels by the same chunked sweep over the output.
+// Where a step cannot run (a position order that is not topological, a table
+// or dirty-set overflow, too many rounds) it falls back to dev_strash(comb);
+// info->final_strash tells which path ran.
+int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &out, int32_t *depthOut,
+                  gls_op_info *info, const uint32_t *tagIn, uint32_t **tagOut, uint32_t *tagDepth) {
+    const int32_t N = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs, n = d.n;
+    hipStream_t st = ctx->stream;
+    if (!d.g0 || !d.g1 || !d.glevel || !d.tOff || !d.tEnt || n > N)
+        return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: the rewrite's input graph is not available");
+    auto fallback = [&](const char *why) {
+        (void)why;
+        const int rc = dev_strash(ctx, in, out, depthOut, info, tagIn, tagOut, tagDepth);
+        if (info) info->final_strash = 2;
+        return rc;
+    };
+    {
+        Tmp t{ctx};
+        uint32_t *ref = t.get<uint32_t>(N), *order = t.get<uint32_t>(N);
+        uint8_t *dead = t.get<uint8_t>(N), *surv = t.get<uint8_t>(N);
+        uint32_t *posOf = t.get<uint32_t>(N), *fa = t.get<uint32_t>(N), *fb = t.get<uint32_t>(N);
+        uint32_t *levPos = t.get<uint32_t>(N), *wave = t.get<uint32_t>(N), *rep = t.get<uint32_t>(N);
+        uint32_t *inD = t.get<uint32_t>(N), *dlist = t.get<uint32_t>(N);
+        uint32_t *survIds = t.get<uint32_t>(N), *survWave = t.get<uint32_t>(N);
+        uint32_t *sortIds = t.get<uint32_t>(N), *sortWave = t.get<uint32_t>(N), *newId = t.get<uint32_t>(N);
+        uint32_t *d_misc = t.get<uint32_t>(16);   // [0] err, [1] max level, [2] dirty count, [3] changed,
+                                                  // [4] survivors, [6..7] probes
+        SweepCtl *ctl = t.get<SweepCtl>(1);
+        if (!ref || !order || !dead || !surv || !posOf || !fa || !fb || !levPos || !wave || !rep || !inD || !dlist ||
+            !survIds || !survWave || !sortIds || !sortWave || !newId || !d_misc || !ctl)
+            return gls_fail(ctx, GLS_ERR_NOMEM, "strash: device allocation failed (n=%d)", N);
+        unsigned long long *d_probes = reinterpret_cast<unsigned long long *>(&d_misc[6]);
+        GLS_HIP(ctx, hipMemsetAsync(ref, 0, sizeof(uint32_t) * N, st));
+        GLS_HIP(ctx, hipMemsetAsync(dead, 0, N, st));
+        GLS_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(SweepCtl), st));
+        GLS_HIP(ctx, hipMemsetAsync(d_misc, 0, sizeof(uint32_t) * 16, st));
+        const int gN = grid_for(N);
+        SweepCtl h_ctl{};
+        uint32_t rounds = 0;
+        // 1. dangling cascade
+        k_count_refs<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, nPOs, in.pos, ref);
+        k_seed_zero<<<gN, kBlock, 0, st>>>(N, nPIs, ref, &ctl->curSize, order, nullptr);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_HIP(ctx, hipMemcpyAsync(&h_ctl, ctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, st));
+        GLS_HIP(ctx, hipStreamSynchronize(st));
+        if (h_ctl.curSize > 0) {
+            GLS_TRY(run_sweep(ctx, ctl, 1, [&](uint32_t k, uint32_t hint) {
+                k_dangle_round<<<sweep_grid(hint), kBlock, 0, st>>>(k, ctl, nPIs, in.f0, in.f1, ref, dead, order);
+            }, &h_ctl, &rounds));
+        }
+        // 2. a topological order (keys relaxed from the input levels, then a
+        //    stable sort), and the waves by one chunked sweep over it
+        k_rw_key_init<<<gN, kBlock, 0, st>>>(n, N, d.glevel, fa);
+        uint32_t keyRounds = 0;
+        for (;; ++keyRounds) {
+            if (keyRounds >= kRwRoundLimit) return fallback("key rounds");
+            GLS_HIP(ctx, hipMemsetAsync(&d_misc[5], 0, sizeof(uint32_t), st));
+            k_rw_key_relax<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, fa, &d_misc[5]);
+            GLS_HIP(ctx, hipGetLastError());
+            uint32_t moved = 0;
+            GLS_TRY(stream_copy(ctx, &moved, &d_misc[5], sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (moved == 0) break;
+        }
+        {
+            // order: ids by key, stable (old ids, then new ids, within a key)
+            k_iota_u32<<<gN, kBlock, 0, st>>>(N, levPos);
+            size_t tbk = 0;
+            GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbk, fa, fb, levPos, order, (size_t)N, 0u, 32u, st));
+            void *tmpk = t.get<uint8_t>(std::max<size_t>(tbk, 1));
+            if (!tmpk) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: key sort storage");
+            GLS_HIP(ctx, rocprim::radix_sort_pairs(tmpk, tbk, fa, fb, levPos, order, (size_t)N, 0u, 32u, st));
+        }
+        k_rw_pos<<<gN, kBlock, 0, st>>>(N, order, posOf);
+        k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
+        GLS_HIP(ctx, hipGetLastError());
+        uint32_t misc[16] = {};
+        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+        if (misc[0]) return fallback("positions");
+        k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)N, fa, fb, levPos, &d_misc[0], &d_misc[1]);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+        if (misc[0]) return fallback("sweep order");
+        const int32_t depth = (int32_t)misc[1];
+        k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD,
+                                         dlist, &d_misc[2]);
+        GLS_HIP(ctx, hipGetLastError());
+        if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(N, nPIs, wave, tagIn, tagDepth);
+        // 3. representatives of the dirty set, in rounds
+        uint32_t nD = 0;
+        GLS_TRY(stream_copy(ctx, &nD, &d_misc[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
+        uint64_t cap = 0;
+        unsigned long long *ht = nullptr;
+        uint2 *dkey = nullptr;
+        uint32_t *dslot = nullptr, *nrep = nullptr;
+        uint32_t prevD = 0, r = 0;
+        for (; nD > 0; ++r) {
+            if (r >= kRwRoundLimit) return fallback("rounds");
+            uint64_t want = 1024;
+            while (want < 4ull * nD) want <<= 1;
+            if (want > cap) {
+                cap = want;
+                ht = t.get<unsigned long long>(2 * cap);
+                dkey = t.get<uint2>(cap / 2);
+                dslot = t.get<uint32_t>(cap / 2);
+                nrep = t.get<uint32_t>(cap / 2);
+                if (!ht || !dkey || !dslot || !nrep) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: dirty table");
+            }
+            k_fill_u64<<<grid_for((int64_t)2 * cap), kBlock, 0, st>>>(2 * cap, ht, KEY_EMPTY);
+            GLS_HIP(ctx, hipMemsetAsync(&d_misc[3], 0, sizeof(uint32_t), st));
+            const int gD = grid_for(nD);
+            k_rw_insert<<<gD, kBlock, 0, st>>>(nD, dlist, nPIs, in.f0, in.f1, rep, wave, ht, cap - 1, dkey, dslot,
+                                               nrep, &d_misc[0], d_probes);
+            k_rw_resolve<<<gD, kBlock, 0, st>>>(nD, dlist, n, dead, wave, rep, inD, ht, dkey, dslot, nrep, d.tOff,
+                                                d.tEnt, dlist, &d_misc[2], (uint32_t)N, &d_misc[3], &d_misc[0],
+                                                d_probes);
+            k_rw_pull<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, rep, inD, dlist, &d_misc[2], (uint32_t)N,
+                                             &d_misc[0]);
+            GLS_HIP(ctx, hipGetLastError());
+            GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost));
+            if (misc[0]) return fallback("dirty table");
+            prevD = nD;
+            nD = misc[2];
+            if (misc[3] == 0 && nD == prevD) break;
+        }
+        // 4. survivors in (wave, id) order
+        k_rw_survivors<<<gN, kBlock, 0, st>>>(N, nPIs, dead, rep, surv);
+        size_t tb = 0;
+        rocprim::counting_iterator<uint32_t> ids(0u);
+        GLS_HIP(ctx, rocprim::select(nullptr, tb, ids, surv, survIds, &d_misc[4], (size_t)N, st));
+        void *tmp = t.get<uint8_t>(tb);
+        if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: select storage");
+        GLS_HIP(ctx, rocprim::select(tmp, tb, ids, surv, survIds, &d_misc[4], (size_t)N, st));
+        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+        const uint32_t m = misc[4];
+        const int32_t nNew = nPIs + 1 + (int32_t)m;
+        GLS_TRY(dev_graph_alloc(ctx, out, nNew, nPIs, nPOs));
+        const uint32_t *sorted = survIds;
+        if (m > 0) {
+            k_gather_u32<<<grid_for(m), kBlock, 0, st>>>((int32_t)m, survIds, wave, survWave);
+            size_t tb3 = 0;
+            const uint32_t endBit = bits_for((uint32_t)depth);
+            GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb3, survWave, sortWave, survIds, sortIds, (size_t)m, 0u,
+                                                   endBit, st));
+            void *tmp3 = t.get<uint8_t>(tb3);
+            if (!tmp3) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "strash: sort storage"); }
+            GLS_HIP(ctx, rocprim::radix_sort_pairs(tmp3, tb3, survWave, sortWave, survIds, sortIds, (size_t)m, 0u,
+                                                   endBit, st));
+            sorted = sortIds;
+        }
+        uint32_t *tags = nullptr;
+        if (tagIn && tagOut) {
+            tags = pool_get<uint32_t>(ctx, (size_t)nNew);
+            if (!tags) { dev_graph_free(ctx, out); return gls_fail(ctx, GLS_ERR_NOMEM, "strash: tags"); }
+            GLS_HIP(ctx, hipMemcpyAsync(tags, tagIn, sizeof(uint32_t) * (nPIs + 1), hipMemcpyDeviceToDevice, st));
+            *tagOut = tags;
+        }
+        GLS_HIP(ctx, hipMemsetAsync(out.f0, 0, sizeof(uint32_t) * (nPIs + 1), st));
+        GLS_HIP(ctx, hipMemsetAsync(out.f1, 0, sizeof(uint32_t) * (nPIs + 1), st));
+        k_new_ids<<<grid_for(m + nPIs + 1), kBlock, 0, st>>>(N, nPIs, (int32_t)m, sorted, newId);
+        // fa / fb are free again: they take the output's fanin ids
+        k_rw_emit<<<grid_for(nNew), kBlock, 0, st>>>((int32_t)m, nPIs, sorted, in.f0, in.f1, rep, newId, out.f0,
+                                                     out.f1, fa, fb, tagIn, tags);
+        if (nPOs > 0) k_rw_pos_out<<<grid_for(nPOs), kBlock, 0, st>>>(nPOs, nPIs, in.pos, rep, newId, out.pos);
+        GLS_HIP(ctx, hipMemsetAsync(&d_misc[0], 0, sizeof(uint32_t) * 2, st));
+        k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)nNew, fa, fb, out.level, &d_misc[0], &d_misc[1]);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+        if (misc[0]) {
+            dev_graph_free(ctx, out);
+            if (tags) { pool_put(ctx, tags); *tagOut = nullptr; }
+            return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: output not topological");
+        }
+        if (depthOut) *depthOut = depth;
+        if (info) {
+            info->levels_enumerated = depth;
+            info->hash_probes += (int64_t)((uint64_t)misc[6] | (uint64_t)misc[7] << 32);
+            info->final_strash = 1;
+            info->strash_rounds = (int64_t)r + 1;
+        }
     }
     return GLS_OK;
 }

@@ -62,6 +62,15 @@ typedef struct {
      * probes (compare-and-swap attempts on a slot) plus value updates through
      * a pre-inserted key's stored location */
     int64_t hash_probes;
+    /* gls_rewrite: the cut schedule that ran (its GLS_RW_*_CUTS flag, 0 for
+     * the default three-level launches; a requested persistent schedule that
+     * could not run reports the schedule that replaced it), and the final
+     * strash: 0 = the wave sweep over the composed graph, 1 = incremental
+     * (only what the replacements changed is rehashed), 2 = incremental that
+     * fell back to the wave sweep; strash_rounds = rounds of the incremental
+     * strash */
+    int32_t cut_schedule, final_strash;
+    int64_t strash_rounds;
 } gls_op_info;
 
 /* ---- context ---------------------------------------------------------- */
@@ -123,6 +132,9 @@ int gls_strash(gls_aig *aig, int32_t flags);
 #define GLS_RW_XCD_CUTS 8
 #define GLS_RW_LEVEL_CUTS 16
 #define GLS_RW_PAIR_CUTS 32
+/* GLS_RW_FULL_STRASH: finish with the wave-sweep strash of the composed graph
+ * instead of the incremental one (the results are identical). */
+#define GLS_RW_FULL_STRASH 64
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing

@@ -116,6 +116,14 @@ def test_cut_launch_modes_agree(gpu_ctx, name, spec):
     f.rewrite(cut_mode=culs_amd.GLS_RW_PAIR_CUTS)
     for x in (b, c, d, e, f):
         assert same(a.arrays(), x.arrays())
+    # the schedule that ran (gls_op_info.cut_schedule): every one as requested;
+    # the XCD-local kernel recomputes with the default schedule when its
+    # workgroups were not all placed on one XCD, and says so
+    ran = [x.last_op()["cut_schedule"] for x in (a, b, c, d, e, f)]
+    log("cut schedules that ran: %s" % ran)
+    assert ran[0] == 0 and ran[1] == culs_amd.GLS_RW_DATAFLOW_CUTS and ran[2] == culs_amd.GLS_RW_COOP_CUTS
+    assert ran[3] in (culs_amd.GLS_RW_XCD_CUTS, 0)
+    assert ran[4] == culs_amd.GLS_RW_LEVEL_CUTS and ran[5] == culs_amd.GLS_RW_PAIR_CUTS
 
 
 @pytest.mark.parametrize("name,spec", CASES[:4], ids=[c[0] for c in CASES[:4]])
