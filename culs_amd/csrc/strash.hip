@@ -897,78 +897,97 @@ __global__ void k_map_pos(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict
 // Longest-path levels of a topologically ordered position array, by one
 // workgroup: lev[p] = 0 where fa[p] == NOFAN, else 1 + max(lev[fa[p]],
 // lev[fb[p]]), fanin positions below p.  Positions go in chunks of SW_WG, one
-// per thread: fanins below the chunk are read from lev (written by this
-// workgroup), the others are waited for in LDS, one barrier per round.  Each
-// round completes at least the smallest pending position, so a chunk needs at
-// most SW_WG rounds; a fanin at or above its position sets *err instead.
+// per thread.  The levels of the last SW_RING positions stay in an LDS ring
+// (one word each, SW_NOLEV until known), so a fanin at most SW_RING - SW_WG
+// positions back is read from LDS; older ones from lev in global memory.  A
+// position whose fanin lies in its own chunk waits for that slot in LDS: every
+// fanin sits at a lower position (the same or an earlier wavefront of the
+// workgroup), so each wait ends; it is bounded anyway, and a fanin at or above
+// its position sets *err instead of waiting.
 constexpr int SW_WG = 1024;
+constexpr int SW_RING = 32768;              // 128 KiB of LDS
 constexpr uint32_t NOFAN = 0xFFFFFFFFu;
+constexpr uint32_t SW_NOLEV = 0xFFFFFFFFu;
+constexpr uint32_t SW_SPIN_LIMIT = 1u << 22;
 // rounds of the incremental strash before it gives way to the wave sweep
 constexpr uint32_t kRwRoundLimit = 4096;
+
+__device__ __forceinline__ uint32_t sw_lds_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint32_t *__restrict__ fa,
                                                       const uint32_t *__restrict__ fb, uint32_t *lev,
                                                       uint32_t *err, uint32_t *maxLev) {
-    // two buffers: chunk c uses buffer c & 1 while each thread marks its entry
-    // of the other one unknown for chunk c + 1.  An entry is one word, the
-    // level or NOLEV, so a reader sees either nothing or the final value.
-    constexpr uint32_t NOLEV = 0xFFFFFFFFu;
-    __shared__ uint32_t s_lev[2][SW_WG];
-    __shared__ uint32_t s_max;
+    __shared__ uint32_t ring[SW_RING];
+    __shared__ uint2 s_fan[2][SW_WG];           // fanin positions of this chunk and the next
+    __shared__ uint32_t s_max, s_bad;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_max = 0;
-    s_lev[0][tid] = NOLEV;
-    uint32_t na = tid < nPos ? fa[tid] : NOFAN, nb = tid < nPos ? fb[tid] : NOFAN;
+    if (tid == 0) { s_max = 0; s_bad = 0; }
+    for (uint32_t i = tid; i < (uint32_t)SW_RING; i += SW_WG) ring[i] = SW_NOLEV;
+    s_fan[0][tid] = tid < nPos ? make_uint2(fa[tid], fb[tid]) : make_uint2(NOFAN, NOFAN);
     uint32_t mx = 0;
-    bool anyBad = false;
     __syncthreads();
     for (uint32_t c = 0, h = 0; c < nPos; c += SW_WG, h ^= 1u) {
-        s_lev[h ^ 1u][tid] = NOLEV;
         const uint32_t p = c + tid;
         const bool active = p < nPos;
-        const uint32_t a = na, b = nb;
-        const uint32_t q = p + SW_WG;          // the next chunk's fanins are in flight meanwhile
-        na = q < nPos ? fa[q] : NOFAN;
-        nb = q < nPos ? fb[q] : NOFAN;
+        const uint2 f = s_fan[h][tid];
+        const uint32_t a = f.x, b = f.y;
         const bool leaf = !active || a == NOFAN;
         const bool bad = !leaf && (a >= p || b >= p);
-        bool needA = !leaf && !bad && a >= c, needB = !leaf && !bad && b >= c;
+        // ring slots of this chunk held positions SW_RING earlier, which this
+        // chunk reads from global memory
+        const uint32_t oldest = c > (uint32_t)(SW_RING - SW_WG) ? c - (uint32_t)(SW_RING - SW_WG) : 0u;
         uint32_t la = 0, lb = 0;
-        if (!leaf && !bad && !needA) la = lev[a];
-        if (!leaf && !bad && !needB) lb = lev[b];
-        bool done = leaf || bad;
-        uint32_t val = 0;
-        if (active && leaf) s_lev[h][tid] = 0;
-        // one barrier per round: a value seen in this round or the next is final
-        for (uint32_t round = 0;; ++round) {
-            if (!done) {
-                if (needA) {
-                    const uint32_t x = s_lev[h][a - c];
-                    if (x != NOLEV) { la = x; needA = false; }
-                }
-                if (needB) {
-                    const uint32_t x = s_lev[h][b - c];
-                    if (x != NOLEV) { lb = x; needB = false; }
-                }
-                if (!needA && !needB) {
-                    val = 1u + (la > lb ? la : lb);
-                    s_lev[h][tid] = val;
-                    done = true;
-                }
-            }
-            if (!__syncthreads_or(!done)) break;
-            if (round >= SW_WG) { anyBad = true; break; }   // unreachable for fanins below their position
+        bool needA = false, needB = false;
+        if (!leaf && !bad) {
+            if (a < oldest) la = lev[a]; else needA = true;
+            if (b < oldest) lb = lev[b]; else needB = true;
         }
-        anyBad |= bad;
+        // wait for those (rare) loads here, then load the next chunk's fanins
+        // (staged in LDS at the end of the chunk): their latency overlaps this
+        // chunk's work, and nothing inside the wait loop below waits on them
+        __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
+        const uint32_t q = p + SW_WG;
+        const uint32_t na = q < nPos ? fa[q] : NOFAN;
+        const uint32_t nb = q < nPos ? fb[q] : NOFAN;
+        ring[p & (SW_RING - 1)] = active && leaf ? 0u : SW_NOLEV;
+        __syncthreads();
+        uint32_t val = 0;
+        // the value is published inside the loop, in the iteration that
+        // completes it: a lane of the same wavefront may be waiting for it
+        bool done = leaf || bad;
+        uint32_t spins = 0;
+        while (!done) {
+            if (needA) {
+                const uint32_t x = sw_lds_load(&ring[a & (SW_RING - 1)]);
+                if (x != SW_NOLEV) { la = x; needA = false; }
+            }
+            if (needB) {
+                const uint32_t x = sw_lds_load(&ring[b & (SW_RING - 1)]);
+                if (x != SW_NOLEV) { lb = x; needB = false; }
+            }
+            if (!needA && !needB) {
+                val = 1u + (la > lb ? la : lb);
+                __hip_atomic_store(&ring[p & (SW_RING - 1)], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                done = true;
+            } else if (++spins > SW_SPIN_LIMIT) {
+                atomicOr(&s_bad, 1u);
+                done = true;
+            }
+        }
+        if (bad) atomicOr(&s_bad, 1u);
         if (active) lev[p] = val;
         mx = val > mx ? val : mx;
-        if (__syncthreads_or(anyBad)) break;
+        s_fan[h ^ 1u][tid] = make_uint2(na, nb);
+        __syncthreads();
+        if (s_bad) break;
     }
     atomicMax(&s_max, mx);
     __syncthreads();
     if (tid == 0) {
         atomicMax(maxLev, s_max);
-        if (anyBad) atomicOr(err, 1u);
+        if (s_bad) atomicOr(err, 1u);
     }
 }
 
