@@ -120,6 +120,9 @@ extern "C" int gls_ctx_create(int device, gls_ctx **out) {
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GLS_ERR_HIP; }
         ctx->device = device;
+        // the static buffer of gls_cec_noalloc: its address on this device
+        // (retried by the entry point if this fails)
+        (void)sim_scratch_init(device);
     }
     *out = ctx;
     return GLS_OK;
@@ -618,6 +621,22 @@ extern "C" int gls_cec(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gl
     pool_put(ctx, d_first);
     pool_put(ctx, d_err);
     return rc;
+}
+
+extern "C" int gls_cec_noalloc(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gls_cec_info *out) {
+    if (!a || !b || !out) return GLS_ERR_ARG;
+    gls_ctx *ctx = a->ctx;
+    if (b->ctx != ctx) return gls_fail(ctx, GLS_ERR_ARG, "cec: the AIGs belong to different contexts");
+    if (ctx->device < 0) return gls_fail(ctx, GLS_ERR_NODEV, "host-only context");
+    if (a->nPIs != b->nPIs || a->nPOs != b->nPOs)
+        return gls_fail(ctx, GLS_ERR_ARG, "cec: interfaces differ (%d vs %d PIs, %d vs %d POs)", a->nPIs, b->nPIs,
+                        a->nPOs, b->nPOs);
+    if (a->nPIs > GLS_SIM_EXHAUSTIVE_PIS && nWords < 1)
+        return gls_fail(ctx, GLS_ERR_ARG, "cec: nWords must be positive");
+    // putting a handle on the device would allocate: it must be there already
+    if (!a->dev_valid || !b->dev_valid)
+        return gls_fail(ctx, GLS_ERR_ARG, "cec (no allocation): both AIGs must already be on the device");
+    return dev_sim_check_noalloc(ctx, a->dev, b->dev, nWords, seed, out);
 }
 
 extern "C" int gls_simulate(gls_aig *a, int32_t nWords, uint64_t seed, int32_t exhaustive, uint64_t *po_words) {

@@ -156,5 +156,10 @@ int dev_graph_alloc(gls_ctx *ctx, DevGraph &g, int32_t nObjs, int32_t nPIs, int3
 int dev_sim_check(gls_ctx *ctx, const DevGraph &a, const DevGraph &b, int32_t nWords, uint64_t seed, uint64_t *rows,
                   size_t rowsBytes, unsigned long long *d_first, uint32_t *d_err, gls_cec_info *out);
 int dev_simulate(gls_ctx *ctx, const DevGraph &a, int32_t nWords, uint64_t seed, int32_t exhaustive, uint64_t *h_out);
+// gls_cec_noalloc: the same check on the static per-device buffer
+// (sim_scratch_init caches its address for a device, with that device current)
+int sim_scratch_init(int device);
+int dev_sim_check_noalloc(gls_ctx *ctx, const DevGraph &a, const DevGraph &b, int32_t nWords, uint64_t seed,
+                          gls_cec_info *out);
 // Row bytes dev_sim_check needs for the first batch of words.
 size_t sim_rows_bytes(int64_t objs, int32_t nPIs, int32_t nWords, size_t budget);

@@ -20,6 +20,7 @@
 #include "gls_internal.h"
 #include "dev_common.h"
 #include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -257,4 +258,58 @@ int dev_simulate(gls_ctx *ctx, const DevGraph &a, int32_t nWords, uint64_t seed,
             for (int32_t w = 0; w < cw; w++) h_out[(size_t)po * words + w0 + w] = buf[(size_t)po * cw + w];
     }
     return GLS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free check (gls_cec_noalloc): a static device buffer, one copy
+// per GPU, held until the process ends.  Its host-side name is not a device
+// address: host code uses only the address hipGetSymbolAddress returns for
+// the device that is current, cached per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[GLS_CEC_NOALLOC_BYTES / 8];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// Called once per device with the device current (gls_ctx_create; again from
+// the entry point if that one failed).
+int sim_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return GLS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (g_scratch[device]) return GLS_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GLS_ERR_HIP;
+    g_scratch[device] = p;
+    return GLS_OK;
+}
+
+int dev_sim_check_noalloc(gls_ctx *ctx, const DevGraph &a, const DevGraph &b, int32_t nWords, uint64_t seed,
+                          gls_cec_info *out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return gls_fail(ctx, GLS_ERR_NODEV, "cec: no device");
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        base = g_scratch[ctx->device];
+    }
+    if (!base) {
+        GLS_HIP(ctx, hipSetDevice(ctx->device));
+        if (sim_scratch_init(ctx->device) != GLS_OK)
+            return gls_fail(ctx, GLS_ERR_HIP, "cec: the static simulation buffer has no device address");
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        base = g_scratch[ctx->device];
+    }
+    // [0, 256): first difference and error flag; the rows after
+    unsigned long long *d_first = static_cast<unsigned long long *>(base);
+    uint32_t *d_err = reinterpret_cast<uint32_t *>(static_cast<char *>(base) + 64);
+    uint64_t *rows = reinterpret_cast<uint64_t *>(static_cast<char *>(base) + 256);
+    const size_t rowsBytes = (size_t)GLS_CEC_NOALLOC_BYTES - 256;
+    const int64_t objs = (int64_t)a.nObjs + b.nObjs;
+    if ((size_t)objs * 8 > rowsBytes)
+        return gls_fail(ctx, GLS_ERR_ARG,
+                        "cec (no allocation): %lld objects need %lld bytes per pattern word, more than the "
+                        "%zu-byte static buffer holds", (long long)objs, (long long)objs * 8, rowsBytes);
+    return dev_sim_check(ctx, a, b, nWords, seed, rows, rowsBytes, d_first, d_err, out);
 }

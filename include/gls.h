@@ -237,6 +237,17 @@ typedef struct {
     int64_t patterns;    /* patterns simulated */
 } gls_cec_info;
 int gls_cec(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gls_cec_info *out);
+/* gls_cec without any allocation.  It has no counterpart in the reference;
+ * it is for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  The simulation
+ * rows live in a static device buffer of GLS_CEC_NOALLOC_BYTES (one per GPU,
+ * held until the process ends); both AIGs must already be on the device (any
+ * device operation on a handle puts it there) and together hold at most
+ * (GLS_CEC_NOALLOC_BYTES - 256) / 8 objects (nObjs a + nObjs b), else
+ * GLS_ERR_ARG.  Results equal gls_cec's.  Calls on one GPU share the buffer:
+ * calls that overlap on one GPU race. */
+#define GLS_CEC_NOALLOC_BYTES 134217728
+int gls_cec_noalloc(gls_aig *a, gls_aig *b, int32_t nWords, uint64_t seed, gls_cec_info *out);
 /* PO values of one AIG over nWords * 64 patterns (exhaustive = 1: pattern p
  * assigns PI j bit j of p, for any nPIs; else the seeded random words above):
  * po_words[po * nWords + w], bit b of word w = pattern 64 w + b. */
