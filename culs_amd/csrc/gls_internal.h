@@ -124,7 +124,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depth, 
 // larger id is x, as {first fanin literal, node | second literal's complement
 // << 31}).
 struct RwDelta {
-    int32_t n;
+    int32_t n, depth;   // depth: g's largest level
     const uint32_t *g0, *g1, *glevel;
     const uint32_t *tOff;
     const uint2 *tEnt;

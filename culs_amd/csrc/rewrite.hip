@@ -1898,7 +1898,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         } else {
             // the incremental strash reads the input graph's fanins: g is
             // released only after it
-            const RwDelta delta{n, g.f0, g.f1, g.level, gv.tOff, gv.tEnt};
+            const RwDelta delta{n, (int32_t)levelStart.size() - 2, g.f0, g.f1, g.level, gv.tOff, gv.tEnt};
             rc = dev_strash_rw(ctx, comb, delta, out, depth, &stInfo, tagC, tagged ? tagOut : nullptr,
                                tagged ? tagDepth : nullptr);
             gGuard.free_now();

@@ -911,6 +911,9 @@ constexpr uint32_t SW_NOLEV = 0xFFFFFFFFu;
 constexpr uint32_t SW_SPIN_LIMIT = 1u << 22;
 // rounds of the incremental strash before it gives way to the wave sweep
 constexpr uint32_t kRwRoundLimit = 4096;
+// objects per level of the rewrite's input above which the final strash is
+// the wave sweep (see dev_strash_rw)
+constexpr int64_t kRwSweepWidth = 1024;
 
 __device__ __forceinline__ uint32_t sw_lds_load(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1491,6 +1494,15 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         if (info) info->final_strash = 2;
         return rc;
     };
+    // The level sweeps below run in one workgroup, about 3 ns per object for
+    // the two of them; the wave sweep costs about 2.6 us per level plus its
+    // per-node work.  On wide graphs (many objects per level, as in a batched
+    // arena) the wave sweep is the faster one.
+    if ((int64_t)N > kRwSweepWidth * (int64_t)std::max(d.depth, 1)) {
+        const int rc = dev_strash(ctx, in, out, depthOut, info, tagIn, tagOut, tagDepth);
+        if (info) info->final_strash = 0;
+        return rc;
+    }
     {
         Tmp t{ctx};
         uint32_t *ref = t.get<uint32_t>(N), *order = t.get<uint32_t>(N);

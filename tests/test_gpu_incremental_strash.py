@@ -69,14 +69,18 @@ def test_incremental_equals_full_sweep(gpu_ctx, name, make):
     log("%s: %d rounds in the last pass" % (name, ia["strash_rounds"]))
 
 
-def test_incremental_in_batches(gpu_ctx):
-    """Batched rw uses the incremental strash with tags: members equal their passes alone."""
-    aigs = [gpu_ctx.gen_random(16 + i, 1500 + 700 * i, 8, 300, 100 + i) for i in range(12)]
+@pytest.mark.parametrize("count,window,path", [(12, 30, 1), (48, 1000, 0)], ids=["deep_incremental", "wide_sweep"])
+def test_final_strash_in_batches(gpu_ctx, count, window, path):
+    """Batched rw with tags: a deep narrow arena takes the incremental strash,
+    a wide one (more than 1,024 objects per level) the wave sweep; either way
+    every member equals its pass alone."""
+    aigs = [gpu_ctx.gen_random(16 + i % 12, 1500 + 700 * (i % 12), 8, window, 100 + i) for i in range(count)]
     gpu_ctx.batch_strash(aigs)
     alone = [x.copy() for x in aigs]
     gpu_ctx.batch_rewrite(aigs)
     info = aigs[0].last_op()
-    assert info["final_strash"] == 1, info
+    log("batch window %d: final_strash %d" % (window, info["final_strash"]))
+    assert info["final_strash"] == path, info
     for x, y in zip(aigs, alone):
         y.rewrite(full_strash=True)
         assert same(x.arrays(), y.arrays())
