@@ -544,6 +544,64 @@ __global__ void k_out_pos(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict
     }
 }
 
+// (level << 32 | tag) of every root in level order, for the batch-split check
+__global__ void k_level_tag_keys(uint32_t nRoots, const uint32_t *__restrict__ levels,
+                                 const uint32_t *__restrict__ roots, const uint32_t *__restrict__ tag,
+                                 unsigned long long *keys) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nRoots; i += stride)
+        keys[i] = (unsigned long long)levels[i] << 32 | tag[roots[i]];
+}
+
+// In sorted (level, tag) keys, a key equal to the one BAL_BATCH places
+// before it belongs to a tag with more than BAL_BATCH covers on that level:
+// its tag goes to the list (at most cap entries; *count counts them all).
+__global__ void k_split_tags(uint32_t nRoots, const unsigned long long *__restrict__ sortedKeys, uint32_t *list,
+                             uint32_t cap, uint32_t *count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)BAL_BATCH + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nRoots; i += stride) {
+        const unsigned long long k = sortedKeys[i];
+        // the first place of each offending run past the limit reports it
+        if (k == sortedKeys[i - BAL_BATCH] && (i == BAL_BATCH || sortedKeys[i - BAL_BATCH - 1] != k)) {
+            const uint32_t q = atomicAdd(count, 1u);
+            if (q < cap) list[q] = (uint32_t)k;
+        }
+    }
+}
+
+// the largest set cover length (at least 1)
+__global__ void k_max_cover_len(int32_t n, const uint32_t *__restrict__ coverLen, uint32_t *out) {
+    __shared__ uint32_t s_m;
+    if (threadIdx.x == 0) s_m = 1;
+    __syncthreads();
+    uint32_t m = 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t l = coverLen[v];
+        if (l != UNSET && l > m) m = l;
+    }
+    atomicMax(&s_m, m);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, s_m);
+}
+
+// steps of every reconstruction batch of the wide levels: the largest
+// (cover length - 1) of its roots; batchOff[l] = first batch of level l
+// (UNSET for a level that the persistent workgroup takes)
+__global__ void k_batch_steps(uint32_t nRoots, const uint32_t *__restrict__ levels,
+                              const uint32_t *__restrict__ roots, const uint32_t *__restrict__ coverLen,
+                              const uint32_t *__restrict__ starts, const uint32_t *__restrict__ batchOff,
+                              uint32_t batchLimit, uint32_t *steps) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nRoots; i += stride) {
+        const uint32_t l = levels[i], bo = batchOff[l];
+        if (bo == UNSET) continue;
+        const uint32_t len = coverLen[roots[i]];
+        const uint32_t st = len > 1 ? len - 1 : 0u;
+        if (st) atomicMax(&steps[bo + ((uint32_t)i - starts[l]) / batchLimit], st);
+    }
+}
+
 struct Tmp {
     gls_ctx *ctx;
     std::vector<void *> ptrs;
@@ -568,7 +626,7 @@ uint32_t bits_of(uint32_t x) {
 } // namespace
 
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn,
-                uint32_t **tagOut) {
+                uint32_t **tagOut, std::vector<uint32_t> *splitTags) {
     const bool tagged = tagIn && tagOut;
     const int32_t n = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs;
     hipStream_t st = ctx->stream;
@@ -647,32 +705,70 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     }
     // Batches: 32768 covers of a level at a time.  With tags (a union of
     // independent AIGs) a level is one batch, which gives every AIG the batches
-    // it gets alone as long as none has more than 32768 covers on a level.
-    if (tagged && nRoots > 0) {
-        std::vector<uint32_t> hk(nRoots), hr(nRoots), htag(n);
-        GLS_TRY(stream_copy(ctx, hk.data(), keysSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
-        GLS_TRY(stream_copy(ctx, hr.data(), rootsSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
-        GLS_TRY(stream_copy(ctx, htag.data(), tagIn, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        std::vector<std::pair<uint64_t, uint32_t>> groups;
-        groups.reserve(nRoots);
-        for (uint32_t i = 0; i < nRoots; i++) groups.push_back({(uint64_t)hk[i] << 32 | htag[hr[i]], 0u});
-        std::sort(groups.begin(), groups.end());
-        uint32_t run = 0;
-        for (uint32_t i = 0; i < nRoots; i++) {
-            run = (i > 0 && groups[i].first == groups[i - 1].first) ? run + 1 : 1;
-            if (run > (uint32_t)BAL_BATCH) return GLS_ERR_BATCH_SPLIT;
+    // it gets alone as long as none has more than 32768 covers on a level: the
+    // (level, tag) keys of the roots are sorted on the device, and a tag with
+    // a longer run is reported to the caller (splitTags), which balances it
+    // alone.
+    if (tagged && nRoots > (uint32_t)BAL_BATCH) {
+        unsigned long long *gk = t.get<unsigned long long>(nRoots), *gkS = t.get<unsigned long long>(nRoots);
+        uint32_t *lst = t.get<uint32_t>(1024), *lcnt = t.get<uint32_t>(1);
+        if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: split check");
+        k_level_tag_keys<<<nblocks(nRoots), kB, 0, st>>>(nRoots, keysSorted, rootsSorted, tagIn, gk);
+        size_t tbk = 0;
+        GLS_HIP(ctx, rocprim::radix_sort_keys(nullptr, tbk, gk, gkS, (size_t)nRoots, 0u, 64u, st));
+        void *tmpk = t.get<uint8_t>(tbk);
+        if (!tmpk) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: split check sort");
+        GLS_HIP(ctx, rocprim::radix_sort_keys(tmpk, tbk, gk, gkS, (size_t)nRoots, 0u, 64u, st));
+        GLS_HIP(ctx, hipMemsetAsync(lcnt, 0, sizeof(uint32_t), st));
+        k_split_tags<<<nblocks(nRoots), kB, 0, st>>>(nRoots, gkS, lst, 1024u, lcnt);
+        GLS_HIP(ctx, hipGetLastError());
+        uint32_t nsplit = 0;
+        GLS_TRY(stream_copy(ctx, &nsplit, lcnt, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (nsplit > 0) {
+            if (splitTags) {
+                splitTags->assign(std::min<uint32_t>(nsplit, 1024u), 0u);
+                GLS_TRY(stream_copy(ctx, splitTags->data(), lst, sizeof(uint32_t) * splitTags->size(),
+                                    hipMemcpyDeviceToHost));
+                if (nsplit > 1024u) splitTags->clear();   // too many to list: the caller splits them all
+            }
+            return GLS_ERR_BATCH_SPLIT;
         }
     }
     const uint32_t batchLimit = tagged ? 0xFFFFFFFFu : (uint32_t)BAL_BATCH;
     // cover lengths: the workspace width, and the steps a batch needs (a step
-    // takes two inputs of every unfinished cover and gives back at most one)
+    // takes two inputs of every unfinished cover and gives back at most one),
+    // reduced on the device
     uint32_t maxLen = 1;
-    std::vector<uint32_t> hl(n), hroots(nRoots);
-    GLS_TRY(stream_copy(ctx, hl.data(), coverLen, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    for (int32_t v = 0; v < n; v++)
-        if (hl[v] != UNSET && hl[v] > maxLen) maxLen = hl[v];
-    if (nRoots > 0)
-        GLS_TRY(stream_copy(ctx, hroots.data(), rootsSorted, sizeof(uint32_t) * nRoots, hipMemcpyDeviceToHost));
+    {
+        uint32_t *d_ml = t.get<uint32_t>(1);
+        if (!d_ml) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: cover length");
+        GLS_HIP(ctx, hipMemsetAsync(d_ml, 0, sizeof(uint32_t), st));
+        k_max_cover_len<<<std::min<int64_t>(nblocks(n), 1024), kB, 0, st>>>(n, coverLen, d_ml);
+        GLS_HIP(ctx, hipGetLastError());
+        GLS_TRY(stream_copy(ctx, &maxLen, d_ml, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    // the wide levels' batches and their step counts
+    std::vector<uint32_t> batchOff(levels + 1, UNSET), batchSteps;
+    uint32_t nBatches = 0;
+    for (uint32_t l = 0; l < levels; l++) {
+        const uint32_t cnt = starts[l + 1] - starts[l];
+        if (cnt <= (uint32_t)BAL_WG_MAX) continue;
+        batchOff[l] = nBatches;
+        nBatches += (cnt + std::min(cnt, batchLimit) - 1) / std::min(cnt, batchLimit);
+    }
+    if (nBatches > 0) {
+        uint32_t *d_off = t.get<uint32_t>(levels + 1), *d_st = t.get<uint32_t>(levels + 2);
+        uint32_t *d_steps = t.get<uint32_t>(nBatches);
+        if (!d_off || !d_st || !d_steps) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: batch steps");
+        GLS_TRY(stream_copy(ctx, d_off, batchOff.data(), sizeof(uint32_t) * (levels + 1), hipMemcpyHostToDevice));
+        GLS_TRY(stream_copy(ctx, d_st, starts.data(), sizeof(uint32_t) * (levels + 2), hipMemcpyHostToDevice));
+        GLS_HIP(ctx, hipMemsetAsync(d_steps, 0, sizeof(uint32_t) * nBatches, st));
+        k_batch_steps<<<nblocks(nRoots), kB, 0, st>>>(nRoots, keysSorted, rootsSorted, coverLen, d_st, d_off,
+                                                      batchLimit, d_steps);
+        GLS_HIP(ctx, hipGetLastError());
+        batchSteps.resize(nBatches);
+        GLS_TRY(stream_copy(ctx, batchSteps.data(), d_steps, sizeof(uint32_t) * nBatches, hipMemcpyDeviceToHost));
+    }
 
     // ---- phase 2: reconstruction
     uint64_t cap = 1024;
@@ -721,10 +817,10 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
             continue;
         }
         // one wide level, in batches, one kernel per phase
+        uint32_t bi = batchOff[l];
         for (uint32_t b0 = starts[l]; b0 < starts[l + 1]; b0 += std::min<uint32_t>(starts[l + 1] - b0, batchLimit)) {
             const uint32_t nb = std::min<uint32_t>(starts[l + 1] - b0, batchLimit);
-            uint32_t steps = 0;
-            for (uint32_t i = b0; i < b0 + nb; i++) steps = std::max(steps, hl[hroots[i]] > 1 ? hl[hroots[i]] - 1 : 0u);
+            const uint32_t steps = batchSteps[bi++];
             k_rec_gather<<<nblocks(nb), kB, 0, st>>>(ra, b0, nb);
             for (uint32_t stp = 0; stp < steps; stp++) {
                 k_rec_prepare<<<nblocks(nb), kB, 0, st>>>(ra, nb);

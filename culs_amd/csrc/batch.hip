@@ -268,13 +268,23 @@ int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t u
     if (!dep) return gls_fail(ctx, GLS_ERR_NOMEM, "batch: depth");
     GLS_HIP(ctx, hipMemsetAsync(dep, 0, sizeof(uint32_t) * count, st));
     gls_op_info info = {};
+    std::vector<uint32_t> splitTags;
     int rc = pass == Pass::Strash    ? dev_strash(ctx, U, R, &depth, &info, utag, &rtag, dep)
              : pass == Pass::Rewrite ? dev_rewrite(ctx, U, useZeros, flags, R, &depth, &info, utag, &rtag, dep)
-                                     : dev_balance(ctx, U, R, &info, utag, &rtag);
+                                     : dev_balance(ctx, U, R, &info, utag, &rtag, &splitTags);
     if (rtag) pb.ptrs.push_back(rtag);
     if (rc == GLS_ERR_BATCH_SPLIT) {
-        // an AIG has a level wider than one balance batch: the members go alone
-        for (int32_t k = 0; k < count; k++) GLS_TRY(gls_balance(aigs[k], 1, 0));
+        // members with a level wider than one balance batch go alone (their
+        // batches are then the ones they get alone); the others stay together
+        std::vector<char> alone(count, splitTags.empty() ? 1 : 0);
+        for (uint32_t tg : splitTags)
+            if (tg < (uint32_t)count) alone[tg] = 1;
+        std::vector<gls_aig *> rest;
+        for (int32_t k = 0; k < count; k++) {
+            if (alone[k]) GLS_TRY(gls_balance(aigs[k], 1, 0));
+            else rest.push_back(aigs[k]);
+        }
+        if (!rest.empty()) GLS_TRY(batch_pass(ctx, rest.data(), (int32_t)rest.size(), Pass::Balance, 0, 0));
         return GLS_OK;
     }
     if (rc != GLS_OK) return rc;

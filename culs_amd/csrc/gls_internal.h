@@ -138,11 +138,12 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
                 gls_op_info *info, const uint32_t *tagIn = nullptr, uint32_t **tagOut = nullptr,
                 uint32_t *tagDepth = nullptr);
 // tagIn / tagOut as in dev_strash.  Returns GLS_ERR_BATCH_SPLIT (without a
-// message) when one tag has more than 32768 covers on a level: that tag would
-// be rebuilt in other batches than alone.
+// message) when a tag has more than 32768 covers on a level: that tag would
+// be rebuilt in other batches than alone.  splitTags (optional) then lists
+// those tags (empty when there are more than 1024 of them).
 #define GLS_ERR_BATCH_SPLIT (-100)
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn = nullptr,
-                uint32_t **tagOut = nullptr);
+                uint32_t **tagOut = nullptr, std::vector<uint32_t> *splitTags = nullptr);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
                        uint32_t *d_cuts, int32_t *d_nref, int mode, const uint8_t *sigBits = nullptr,
                        int *ranMode = nullptr);
