@@ -49,7 +49,8 @@ constexpr int BAL_MAX_COVER = 1024;   // MAX_LOCAL_COVER_SIZE, balance.cu:17
 constexpr int BAL_BATCH = 32768;      // iterLen, balance.cu:1129
 constexpr int BAL_WG = 1024;          // threads of the persistent reconstruction workgroup
 constexpr int BAL_WG_MAX = 4096;      // widest level handled by it
-constexpr int COVER_THREADS = 16384;  // threads of a cover-discovery launch (each owns COVER_SCRATCH slots)
+constexpr int COVER_THREADS = 16384;  // threads of a cover-discovery launch (each owns COVER_SCRATCH slots),
+constexpr int COVER_THREADS_MAX = 131072;   // scaled up to this with the graph (a level's frontier in one round)
 // the size limit is checked before an expansion, so a cover can end up to a
 // few inputs above BAL_MAX_COVER: room for that
 constexpr int COVER_SCRATCH = BAL_MAX_COVER + 64;
@@ -598,7 +599,10 @@ __global__ void k_batch_steps(uint32_t nRoots, const uint32_t *__restrict__ leve
         if (bo == UNSET) continue;
         const uint32_t len = coverLen[roots[i]];
         const uint32_t st = len > 1 ? len - 1 : 0u;
-        if (st) atomicMax(&steps[bo + ((uint32_t)i - starts[l]) / batchLimit], st);
+        // most roots of a batch see a maximum at least as large already: the
+        // atomic is taken only by those that raise it
+        uint32_t *dst = &steps[bo + ((uint32_t)i - starts[l]) / batchLimit];
+        if (st > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, st);
     }
 }
 
@@ -640,7 +644,11 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     uint32_t *flen = t.get<uint32_t>((size_t)n + 2 + COVER_LEVELS_PER_CHECK);   // frontier size per level
     const uint64_t poolCap = 2ull * (uint64_t)n + 64;   // cover inputs are distinct fanin edges
     uint32_t *pool = t.get<uint32_t>(poolCap);
-    uint32_t *scratch = t.get<uint32_t>((size_t)COVER_THREADS * COVER_SCRATCH);
+    // one cover-discovery thread per 8 objects within the bounds: a wide
+    // level's frontier is then walked in one or two rounds per thread
+    const int coverThreads = (int)std::min<int64_t>(
+        COVER_THREADS_MAX, std::max<int64_t>(COVER_THREADS, ((int64_t)n / 8 + kB - 1) / kB * kB));
+    uint32_t *scratch = t.get<uint32_t>((size_t)coverThreads * COVER_SCRATCH);
     if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "balance: allocation (n=%d)", n);
     GLS_HIP(ctx, hipMemsetAsync(nFo, 0, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(coverLen, 0xFF, sizeof(uint32_t) * n, st));
@@ -659,7 +667,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     for (uint32_t base = 0;; base += COVER_LEVELS_PER_CHECK) {
         if ((uint64_t)base > (uint64_t)n) return gls_fail(ctx, GLS_ERR_INTERNAL, "balance: cover levels do not end");
         for (uint32_t l = base; l < base + COVER_LEVELS_PER_CHECK; l++)
-            k_cover_level<<<COVER_THREADS / kB, kB, 0, st>>>(ca, frontA, frontB, flen, l);
+            k_cover_level<<<coverThreads / kB, kB, 0, st>>>(ca, frontA, frontB, flen, l);
         GLS_HIP(ctx, hipGetLastError());
         uint32_t h[COVER_LEVELS_PER_CHECK + 1];
         GLS_TRY(stream_copy(ctx, h, &flen[base], sizeof(h), hipMemcpyDeviceToHost));

@@ -241,6 +241,29 @@ int aig_ensure_host(gls_aig *a) {
     return GLS_OK;
 }
 
+// A handle over arrays that are already AIGER literals with fanins below
+// their node (checked by the caller); the vectors are taken over.
+gls_aig *aig_adopt(gls_ctx *ctx, int32_t nPIs, std::vector<uint32_t> &f0, std::vector<uint32_t> &f1,
+                   std::vector<uint32_t> &pos) {
+    return new_aig(ctx, nPIs, f0, f1, pos);
+}
+
+// Allocates the device copy and queues its upload on the context stream; the
+// caller synchronises the stream before the host arrays change.
+int aig_upload_async(gls_aig *a) {
+    gls_ctx *ctx = a->ctx;
+    if (ctx->device < 0) return gls_fail(ctx, GLS_ERR_NODEV, "host-only context");
+    if (a->dev_valid) return GLS_OK;
+    GLS_TRY(dev_graph_alloc(ctx, a->dev, a->nObjs, a->nPIs, a->nPOs));
+    GLS_HIP(ctx, hipMemcpyAsync(a->dev.f0, a->h_f0.data(), sizeof(uint32_t) * a->nObjs, hipMemcpyHostToDevice, ctx->stream));
+    GLS_HIP(ctx, hipMemcpyAsync(a->dev.f1, a->h_f1.data(), sizeof(uint32_t) * a->nObjs, hipMemcpyHostToDevice, ctx->stream));
+    if (a->nPOs)
+        GLS_HIP(ctx, hipMemcpyAsync(a->dev.pos, a->h_pos.data(), sizeof(uint32_t) * a->nPOs, hipMemcpyHostToDevice, ctx->stream));
+    a->dev_valid = true;
+    a->dev_level_valid = false;
+    return GLS_OK;
+}
+
 int aig_ensure_device(gls_aig *a) {
     gls_ctx *ctx = a->ctx;
     if (ctx->device < 0) return gls_fail(ctx, GLS_ERR_NODEV, "host-only context");

@@ -99,6 +99,9 @@ int host_strash_cpu(int32_t nPIs, const std::vector<uint32_t> &f0, const std::ve
 
 // host <-> device
 int aig_ensure_device(gls_aig *aig);
+gls_aig *aig_adopt(gls_ctx *ctx, int32_t nPIs, std::vector<uint32_t> &f0, std::vector<uint32_t> &f1,
+                   std::vector<uint32_t> &pos);
+int aig_upload_async(gls_aig *aig);
 int aig_ensure_host(gls_aig *aig);
 void aig_drop_device(gls_aig *aig);
 int ctx_ensure_library(gls_ctx *ctx);

@@ -114,16 +114,23 @@ void gls_server::run() {
                 pending.pop_front();
             }
         }
-        // upload the admitted jobs: they stay on the device until done
+        // upload the admitted jobs (arrays checked and converted at submit,
+        // taken over by the handle; one stream synchronisation for all of
+        // them): they stay on the device until done
         for (auto &j : admitted) {
-            const int32_t nAnds = (int32_t)j->f0.size() - j->nPIs - 1;
-            int rc = gls_aig_from_arrays(ctx, j->nPIs, (int32_t)j->pos.size(), nAnds, j->f0.data(), j->f1.data(),
-                                         j->pos.empty() ? nullptr : j->pos.data(), GLS_LIT_AIGER, &j->aig);
-            if (rc == GLS_OK) rc = aig_ensure_device(j->aig);
+            j->aig = aig_adopt(ctx, j->nPIs, j->f0, j->f1, j->pos);
+            const int rc = aig_upload_async(j->aig);
             if (rc != GLS_OK) { j->wstatus = rc; j->pc = j->cmds.size(); }
-            std::vector<uint32_t>().swap(j->f0);
-            std::vector<uint32_t>().swap(j->f1);
         }
+        if (!admitted.empty() && hipStreamSynchronize(ctx->stream) != hipSuccess)
+            for (auto &j : admitted) { j->wstatus = GLS_ERR_HIP; j->pc = j->cmds.size(); }
+        for (auto &j : admitted)
+            if (j->aig) {
+                // the host copy is stale once a pass runs; free it now
+                std::vector<uint32_t>().swap(j->aig->h_f0);
+                std::vector<uint32_t>().swap(j->aig->h_f1);
+                j->aig->host_valid = false;
+            }
         // one batched pass per command kind over the jobs whose next command it is
         std::vector<gls_aig *> group[CMD_KINDS];
         std::vector<gls_job *> owners[CMD_KINDS];
