@@ -176,6 +176,7 @@ struct CoverArgs {
     uint64_t poolCap;
     uint32_t *scratch;               // COVER_SCRATCH entries per thread
     uint32_t *lastLevel, *stamp;
+    uint32_t *indeg;                 // parent covers of a root not yet taken by the topological sweep
     uint32_t *err;                   // 1: cover pool full
 };
 
@@ -208,38 +209,76 @@ __device__ uint32_t cover_dfs(const CoverArgs &a, uint32_t v, uint32_t *in) {
     return len;
 }
 
-// one level of the root sweep: record the level, compute covers seen for the
-// first time, and put their AND inputs on the next frontier (each once)
-// flen[level] = size of the level's frontier, kept on the device so that
-// several levels are launched per host check; level L's frontier is in
-// fronts[L & 1].
-__global__ void k_cover_level(CoverArgs a, uint32_t *front0, uint32_t *front1, uint32_t *flen, uint32_t level) {
-    const uint32_t frontLen = flen[level];
-    const uint32_t *front = (level & 1u) ? front1 : front0;
-    uint32_t *next = (level & 1u) ? front0 : front1;
-    uint32_t *nextLen = &flen[level + 1];
+// Root levels.  The reference sweeps levels from the POs, putting on level
+// L + 1 every AND input of a cover on level L (again on every level it is
+// reached on), and keeps for each root the last level it appeared on
+// (balance.cu:958-1037): a root's level is the length of the longest cover
+// path from a PO to it, and the number of levels is one more than the
+// longest.  Here each root is visited twice instead of once per path length:
+//   reach: a sweep from the POs in which each root is put on a frontier the
+//          first time it is seen; its cover is computed then, and each AND
+//          input of the cover counts one more parent cover;
+//   kahn:  a synchronous topological sweep of the cover graph: round 0 holds
+//          the roots without a parent cover (PO drivers), and a root goes on
+//          the next round's frontier when its last parent cover is taken.
+// A root taken in round L has its longest path from a PO equal to L.
+// flen[r] = size of round r's frontier, kept on the device so that several
+// rounds are launched per host check; round r's frontier is in
+// fronts[r & 1].
+__global__ void k_cover_reach(CoverArgs a, uint32_t *front0, uint32_t *front1, uint32_t *flen, uint32_t round) {
+    const uint32_t frontLen = flen[round];
+    const uint32_t *front = (round & 1u) ? front1 : front0;
+    uint32_t *next = (round & 1u) ? front0 : front1;
+    uint32_t *nextLen = &flen[round + 1];
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     uint32_t *in = a.scratch + (size_t)tid * COVER_SCRATCH;
     for (int64_t k = tid; k < frontLen; k += stride) {
         const uint32_t v = front[k];
-        a.lastLevel[v] = level;
-        uint32_t len = a.coverLen[v], off = a.coverOff[v];
-        if (len == UNSET) {
-            len = cover_dfs(a, v, in);
-            off = atomicAdd(a.poolNext, len);
-            if ((uint64_t)off + len > a.poolCap) {
-                atomicOr(a.err, 1u);
-                a.coverLen[v] = 0;
-                continue;
-            }
-            for (uint32_t i = 0; i < len; i++) a.pool[off + i] = in[i];
-            a.coverOff[v] = off;
-            a.coverLen[v] = len;
+        const uint32_t len = cover_dfs(a, v, in);
+        const uint32_t off = atomicAdd(a.poolNext, len);
+        if ((uint64_t)off + len > a.poolCap) {
+            atomicOr(a.err, 1u);
+            a.coverLen[v] = 0;
+            continue;
         }
         for (uint32_t i = 0; i < len; i++) {
+            const uint32_t lit = in[i], id = lit >> 1;
+            a.pool[off + i] = lit;
+            if ((int32_t)id > a.nPIs) {
+                atomicAdd(&a.indeg[id], 1u);
+                if (atomicExch(&a.stamp[id], 1u) != 1u) next[atomicAdd(nextLen, 1u)] = id;
+            }
+        }
+        a.coverOff[v] = off;
+        a.coverLen[v] = len;
+    }
+}
+
+// round 0 of the topological sweep: the PO drivers that no cover has as input
+__global__ void k_kahn_first(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict__ pos,
+                             const uint32_t *__restrict__ indeg, uint32_t *stamp, uint32_t *front,
+                             uint32_t *frontLen) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nPOs; i += stride) {
+        const uint32_t id = LID(pos[i]);
+        if ((int32_t)id > nPIs && indeg[id] == 0u && atomicExch(&stamp[id], 2u) != 2u)
+            front[atomicAdd(frontLen, 1u)] = id;
+    }
+}
+
+__global__ void k_cover_kahn(CoverArgs a, uint32_t *front0, uint32_t *front1, uint32_t *flen, uint32_t round) {
+    const uint32_t frontLen = flen[round];
+    const uint32_t *front = (round & 1u) ? front1 : front0;
+    uint32_t *next = (round & 1u) ? front0 : front1;
+    uint32_t *nextLen = &flen[round + 1];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < frontLen; k += stride) {
+        const uint32_t v = front[k];
+        a.lastLevel[v] = round;
+        const uint32_t len = a.coverLen[v], off = a.coverOff[v];
+        for (uint32_t i = 0; i < len; i++) {
             const uint32_t id = a.pool[off + i] >> 1;
-            if ((int32_t)id > a.nPIs && atomicExch(&a.stamp[id], level + 2u) != level + 2u)
-                next[atomicAdd(nextLen, 1u)] = id;
+            if ((int32_t)id > a.nPIs && atomicSub(&a.indeg[id], 1u) == 1u) next[atomicAdd(nextLen, 1u)] = id;
         }
     }
 }
@@ -638,7 +677,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     const auto t0 = std::chrono::steady_clock::now();
     // ---- phase 1: covers and root levels
     uint32_t *nFo = t.get<uint32_t>(n), *coverOff = t.get<uint32_t>(n), *coverLen = t.get<uint32_t>(n);
-    uint32_t *lastLevel = t.get<uint32_t>(n), *stamp = t.get<uint32_t>(n);
+    uint32_t *lastLevel = t.get<uint32_t>(n), *stamp = t.get<uint32_t>(n), *indeg = t.get<uint32_t>(n);
     uint32_t *frontA = t.get<uint32_t>(n), *frontB = t.get<uint32_t>(n);
     uint32_t *ctr = t.get<uint32_t>(8);   // [1] pool next, [2] err, [4..7] reconstruction counter
     uint32_t *flen = t.get<uint32_t>((size_t)n + 2 + COVER_LEVELS_PER_CHECK);   // frontier size per level
@@ -654,28 +693,38 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     GLS_HIP(ctx, hipMemsetAsync(coverLen, 0xFF, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(lastLevel, 0xFF, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(stamp, 0, sizeof(uint32_t) * n, st));
+    GLS_HIP(ctx, hipMemsetAsync(indeg, 0, sizeof(uint32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(ctr, 0, sizeof(uint32_t) * 8, st));
     GLS_HIP(ctx, hipMemsetAsync(flen, 0, sizeof(uint32_t) * ((size_t)n + 2 + COVER_LEVELS_PER_CHECK), st));
     k_fanouts<<<nblocks(n), kB, 0, st>>>(n, nPIs, in.f0, in.f1, nPOs, in.pos, nFo);
     if (nPOs > 0) k_po_frontier<<<nblocks(nPOs), kB, 0, st>>>(nPOs, nPIs, in.pos, stamp, frontA, &flen[0]);
     GLS_HIP(ctx, hipGetLastError());
     CoverArgs ca{in.f0, in.f1, nFo, nPIs, coverOff, coverLen, pool, &ctr[1], poolCap, scratch, lastLevel, stamp,
-                 &ctr[2]};
-    // levels are launched COVER_LEVELS_PER_CHECK at a time; a launch past the
-    // last level finds an empty frontier and does nothing
-    uint32_t levels = 0;
-    for (uint32_t base = 0;; base += COVER_LEVELS_PER_CHECK) {
-        if ((uint64_t)base > (uint64_t)n) return gls_fail(ctx, GLS_ERR_INTERNAL, "balance: cover levels do not end");
-        for (uint32_t l = base; l < base + COVER_LEVELS_PER_CHECK; l++)
-            k_cover_level<<<coverThreads / kB, kB, 0, st>>>(ca, frontA, frontB, flen, l);
-        GLS_HIP(ctx, hipGetLastError());
-        uint32_t h[COVER_LEVELS_PER_CHECK + 1];
-        GLS_TRY(stream_copy(ctx, h, &flen[base], sizeof(h), hipMemcpyDeviceToHost));
-        int first0 = -1;
-        for (int i = 0; i <= COVER_LEVELS_PER_CHECK; i++)
-            if (h[i] == 0) { first0 = i; break; }
-        if (first0 >= 0) { levels = base + (uint32_t)first0; break; }
-    }
+                 indeg, &ctr[2]};
+    // rounds are launched COVER_LEVELS_PER_CHECK at a time; a launch past the
+    // last round finds an empty frontier and does nothing.  Returns the
+    // number of rounds with a non-empty frontier.
+    auto sweep = [&](bool kahn, uint32_t &rounds) -> int {
+        for (uint32_t base = 0;; base += COVER_LEVELS_PER_CHECK) {
+            if ((uint64_t)base > (uint64_t)n) return gls_fail(ctx, GLS_ERR_INTERNAL, "balance: cover levels do not end");
+            for (uint32_t r = base; r < base + COVER_LEVELS_PER_CHECK; r++) {
+                if (kahn) k_cover_kahn<<<coverThreads / kB, kB, 0, st>>>(ca, frontA, frontB, flen, r);
+                else k_cover_reach<<<coverThreads / kB, kB, 0, st>>>(ca, frontA, frontB, flen, r);
+            }
+            GLS_HIP(ctx, hipGetLastError());
+            uint32_t h[COVER_LEVELS_PER_CHECK + 1];
+            GLS_TRY(stream_copy(ctx, h, &flen[base], sizeof(h), hipMemcpyDeviceToHost));
+            for (int i = 0; i <= COVER_LEVELS_PER_CHECK; i++)
+                if (h[i] == 0) { rounds = base + (uint32_t)i; return GLS_OK; }
+        }
+    };
+    uint32_t reachRounds = 0, levels = 0;
+    GLS_TRY(sweep(false, reachRounds));
+    // every reached root now has its cover and its count of parent covers
+    GLS_HIP(ctx, hipMemsetAsync(flen, 0, sizeof(uint32_t) * ((size_t)reachRounds + 2), st));
+    if (nPOs > 0) k_kahn_first<<<nblocks(nPOs), kB, 0, st>>>(nPOs, nPIs, in.pos, indeg, stamp, frontA, &flen[0]);
+    GLS_HIP(ctx, hipGetLastError());
+    GLS_TRY(sweep(true, levels));
     uint32_t herr = 0, poolUsed = 0;
     GLS_TRY(stream_copy(ctx, &herr, &ctr[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
     GLS_TRY(stream_copy(ctx, &poolUsed, &ctr[1], sizeof(uint32_t), hipMemcpyDeviceToHost));
