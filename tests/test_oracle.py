@@ -75,7 +75,9 @@ def test_strash_invariants(host_ctx, spec):
         assert a >= 2 and (a >> 1) != (b >> 1)          # no constant, no x&x, no x&!x
         assert (a, b) not in keys
         keys.add((a, b))
-    # idempotent
+    # a second strash changes nothing or only shrinks the graph: dangling
+    # nodes are removed before hashing (strash.cu:175-188), so nodes left
+    # dangling by this pass's merges and folds go in the next one
     s2, _ = O.strash(*s)
     assert all(np.array_equal(x, y) for x, y in zip(s, s2)) or O.n_ands(s2) <= O.n_ands(s)
     assert O.depth(nPIs, f0, f1) <= depth
