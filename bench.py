@@ -52,6 +52,7 @@ TRAFFIC_NOTE = ("profiled constant of the three-level cut kernel (profiles/k_cut
 def cut_kernel_name(mode):
     """The cut-enumeration kernel that a rewrite with this schedule launches."""
     import culs_amd
+    mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     return {culs_amd.GLS_RW_XCD_CUTS: "k_cuts_xcd", culs_amd.GLS_RW_LEVEL_CUTS: "k_cuts_level",
             culs_amd.GLS_RW_COOP_CUTS: "k_cuts_levels", culs_amd.GLS_RW_DATAFLOW_CUTS: "k_cuts_dataflow",
             culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair"}.get(mode, "k_cuts_triple")
@@ -60,6 +61,7 @@ def cut_kernel_name(mode):
 def cut_launches(mode, nlev):
     """Cut-enumeration launches per rw pass over nlev AND levels."""
     import culs_amd
+    mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     if mode in (culs_amd.GLS_RW_XCD_CUTS, culs_amd.GLS_RW_COOP_CUTS, culs_amd.GLS_RW_DATAFLOW_CUTS):
         return 1
     per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2}.get(mode, 3)
@@ -91,7 +93,9 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline runs (median)")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cut-mode", type=int, default=0,
-                    help="cut-enumeration schedule: 0 per-level launches, 2 cooperative, 4 dataflow")
+                    help="cut-enumeration schedule (gls_rewrite flags): 0 three levels per launch, 2 "
+                         "cooperative, 4 dataflow, 8 one XCD, 16 per level, 32 two levels per launch; +128 "
+                         "launches the schedule from one captured HIP graph")
     ap.add_argument("--stage", choices=["rw", "cuts", "strash"], default="rw",
                     help="cuts: time only reorder + cut enumeration (gls_enumerate_cuts); strash: only the "
                          "strash of the generated input; both for counter runs")
@@ -286,6 +290,10 @@ def main():
                 "unit": "Mnodes/s", "strash_ms": round(strash_ms, 3), "rw_ms": round(ms_per_step, 3)},
         "hash": hash_info,
     }
+    if info["cut_graph_nodes"]:
+        res["cut_graph"] = {"capture_instantiate_ms": round(info["cut_graph_ms"], 3),
+                            "kernel_nodes": info["cut_graph_nodes"],
+                            "note": "cuts_ms holds capture, instantiation and the graph's execution"}
     if cpu is not None and cpu.get("e2e_value"):
         res["e2e"]["vs_cpu"] = round(res["e2e"]["value"] / cpu["e2e_value"], 3)
     if cpu is not None:

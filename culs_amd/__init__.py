@@ -22,6 +22,7 @@ GLS_RW_XCD_CUTS = 8
 GLS_RW_LEVEL_CUTS = 16
 GLS_RW_PAIR_CUTS = 32
 GLS_RW_FULL_STRASH = 64
+GLS_RW_GRAPH_CUTS = 128
 
 
 class GlsError(RuntimeError):
@@ -52,7 +53,8 @@ class OpInfo(ctypes.Structure):
                 ("kernel_sample_ms", ctypes.c_double), ("kernel_samples", ctypes.c_int64),
                 ("kernel_sample_nodes", ctypes.c_int64), ("hash_probes", ctypes.c_int64),
                 ("cut_schedule", ctypes.c_int32), ("final_strash", ctypes.c_int32),
-                ("strash_rounds", ctypes.c_int64)]
+                ("strash_rounds", ctypes.c_int64), ("cut_graph_ms", ctypes.c_double),
+                ("cut_graph_nodes", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -279,8 +281,10 @@ class Aig:
         enumeration: 0 one launch per three levels, GLS_RW_PAIR_CUTS per two,
         GLS_RW_LEVEL_CUTS per level, GLS_RW_COOP_CUTS one cooperative launch,
         GLS_RW_DATAFLOW_CUTS fanin-driven, GLS_RW_XCD_CUTS one persistent
-        kernel on one XCD; full_strash finishes with the wave-sweep strash
-        instead of the incremental one.  The results are identical."""
+        kernel on one XCD; GLS_RW_GRAPH_CUTS or-ed into the first three
+        captures the pass's cut launches in one HIP graph; full_strash
+        finishes with the wave-sweep strash instead of the incremental one.
+        The results are identical."""
         flags = cut_mode | (GLS_RW_FULL_STRASH if full_strash else 0)
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
         return self

@@ -61,6 +61,9 @@ struct gls_ctx {
     std::vector<int64_t> ksamp_units; // units (nodes) of each sampled window
     std::vector<int64_t> ksamp_launches; // launches in each sampled window
     int ksamp_n = 0;
+    // host time of the last cut-graph capture + instantiation, and its launches
+    double cut_graph_ms = 0;
+    int64_t cut_graph_nodes = 0;
 };
 
 // A device-resident AIG in the AIGER literal convention.

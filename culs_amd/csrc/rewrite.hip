@@ -1511,6 +1511,10 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     ctx->ksamp_n = 0;
     ctx->ksamp_units.clear();
     ctx->ksamp_launches.clear();
+    ctx->cut_graph_ms = 0;
+    ctx->cut_graph_nodes = 0;
+    const bool graphed = (mode & GLS_RW_GRAPH_CUTS) != 0;
+    mode &= ~GLS_RW_GRAPH_CUTS;
     GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
     GLS_HIP(ctx, hipMemsetAsync(d_cuts, 0, sizeof(uint32_t) * (size_t)n * NCUT * 5, st));
     k_nref<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, g.f0, g.f1, g.nPOs, g.pos, d_nref);
@@ -1653,12 +1657,19 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // stop event, so the elapsed time over the launches is the per-launch
         // kernel time a profiler reports, with the events' own cost spread over
         // the window
+        // GLS_RW_GRAPH_CUTS: the same launches captured into a HIP graph (kernel
+        // nodes only), instantiated and launched once; the host time of the
+        // capture and the instantiation is reported apart
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const auto tg0 = std::chrono::steady_clock::now();
+        if (graphed) GLS_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int win = 0;
         for (size_t i = 0; i < plan.size(); i++) {
             const CutLaunch &c = plan[i];
             const dim3 grid(c.kind > 1 ? (unsigned)(c.e - c.s) : blocks(c.e - c.s, CUTL_BLOCK / 64));
             hipEvent_t evStart = nullptr, evStop = nullptr;
-            if (kSampleLaunches && win == 0 && c.kind == step && i % KSAMPLE_EVERY == 0) {
+            if (kSampleLaunches && !graphed && win == 0 && c.kind == step && i % KSAMPLE_EVERY == 0) {
                 const size_t k = 2 * (size_t)ctx->ksamp_n;
                 while (ctx->kev.size() < k + 2) {
                     hipEvent_t ev = nullptr;
@@ -1698,6 +1709,25 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
             } else {
                 k_cuts_level<<<grid, CUTL_BLOCK, 0, st>>>(c.s, c.e - c.s, g.f0, g.f1, d_nref, d_cuts, sigBits);
             }
+        }
+        if (graphed) {
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (ec != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cut graph capture: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                hipGraphDestroy(graph);
+                return gls_fail(ctx, GLS_ERR_HIP, "cut graph instantiation: %s", hipGetErrorString(ei));
+            }
+            ctx->cut_graph_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg0).count();
+            ctx->cut_graph_nodes = (int64_t)plan.size();
+            hipError_t el = hipGraphLaunch(exec, st);
+            // the executable graph is released once its launch has run
+            if (el == hipSuccess) el = hipStreamSynchronize(st);
+            hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+            if (el != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cut graph launch: %s", hipGetErrorString(el));
+            ran |= GLS_RW_GRAPH_CUTS;
         }
     }
     GLS_HIP(ctx, hipGetLastError());
@@ -1787,7 +1817,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         int cutRan = 0;
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
                                    flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS |
-                                            GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS),
+                                            GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS | GLS_RW_GRAPH_CUTS),
                                    sigBits, &cutRan));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr};
@@ -1925,6 +1955,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->final_strash = stInfo.final_strash;
             info->cut_schedule = cutRan;
             info->strash_rounds = stInfo.strash_rounds;
+            info->cut_graph_ms = ctx->cut_graph_ms;
+            info->cut_graph_nodes = ctx->cut_graph_nodes;
             double kms = 0;
             int64_t kunits = 0, klaunches = 0;
             for (int i = 0; i < ctx->ksamp_n; i++) {

@@ -71,6 +71,11 @@ typedef struct {
      * strash */
     int32_t cut_schedule, final_strash;
     int64_t strash_rounds;
+    /* gls_rewrite with GLS_RW_GRAPH_CUTS: host time of capturing the cut
+     * launches into a HIP graph and instantiating it, and the kernel nodes
+     * of that graph (cuts_ms covers capture, instantiation and execution) */
+    double cut_graph_ms;
+    int64_t cut_graph_nodes;
 } gls_op_info;
 
 /* ---- context ---------------------------------------------------------- */
@@ -135,6 +140,10 @@ int gls_strash(gls_aig *aig, int32_t flags);
 /* GLS_RW_FULL_STRASH: finish with the wave-sweep strash of the composed graph
  * instead of the incremental one (the results are identical). */
 #define GLS_RW_FULL_STRASH 64
+/* GLS_RW_GRAPH_CUTS: with the default, pair or per-level schedule, capture the
+ * pass's cut launches into a HIP graph and launch that once (for measuring
+ * graph launch against stream launches; the results are identical). */
+#define GLS_RW_GRAPH_CUTS 128
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing

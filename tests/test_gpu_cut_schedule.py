@@ -6,7 +6,8 @@ for any group of three levels holding a level wider than 2048 nodes.  These
 layered AIGs place wide and narrow levels, widths on both sides of that limit
 and long runs of one-node levels at every offset within a group, so groups of
 every shape occur: the cut records must equal the oracle's, and rw must give
-the same AIG under every schedule and equal the oracle's rw.
+the same AIG under every schedule, launched from the stream or from one
+captured HIP graph, and equal the oracle's rw.
 """
 import sys
 
@@ -76,9 +77,14 @@ def test_cut_schedule_parity(gpu_ctx, name, spec):
         int(np.argmax((cuts != want_cuts).reshape(-1, 40).any(axis=1))))
     want, _ = O.rw(*s)
     outs = []
-    for mode in (0, culs_amd.GLS_RW_LEVEL_CUTS, culs_amd.GLS_RW_PAIR_CUTS):
+    G = culs_amd.GLS_RW_GRAPH_CUTS
+    for mode in (0, culs_amd.GLS_RW_LEVEL_CUTS, culs_amd.GLS_RW_PAIR_CUTS, G, G | culs_amd.GLS_RW_LEVEL_CUTS):
         x = a.copy()
         x.rewrite(cut_mode=mode)
+        info = x.last_op()
+        assert info["cut_schedule"] == mode
+        if mode & G:
+            assert info["cut_graph_nodes"] > 0 and info["cut_graph_ms"] > 0
         outs.append(x.arrays())
     for o in outs:
         assert all(np.array_equal(p, q) for p, q in zip(o, want)), "GPU rw differs from the oracle"
