@@ -1200,18 +1200,42 @@ __global__ void k_rw_survivors(int32_t N, int32_t nPIs, const uint8_t *__restric
     for (int64_t v = tid; v < N; v += stride) surv[v] = v > nPIs && !dead[v] && rep[v] == 2u * (uint32_t)v;
 }
 
+// Sets *shorter if some live node's representative lies on a lower wave than
+// the node (a merge into a node of a lower wave, or a fold to a fanin, a PI or
+// a constant).  If none does, every survivor's output level is its wave: its
+// output fanins are representatives on the waves of its composed fanins.
+__global__ void k_rw_shorter(int32_t N, int32_t nPIs, const uint8_t *__restrict__ dead,
+                             const uint32_t *__restrict__ rep, const uint32_t *__restrict__ wave, uint32_t *shorter) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    bool any = false;
+    for (int64_t v = nPIs + 1 + tid; v < N; v += stride) {
+        if (dead[v]) continue;
+        const uint32_t r = LID(rep[v]);
+        if (r == (uint32_t)v) continue;
+        const uint32_t wr = r <= (uint32_t)nPIs ? 0u : wave[r];
+        any |= wr != wave[v];
+    }
+    if (__ballot(any) != 0 && (threadIdx.x & 63) == 0) atomicOr(shorter, 1u);
+}
+
 __device__ __forceinline__ uint32_t out_lit(uint32_t lit, int32_t nPIs, const uint32_t *rep, const uint32_t *newId) {
     const uint32_t r = rep_lit(rep, nPIs, lit);
     return 2u * newId[LID(r)] | LC(r);
 }
 
-// Output fanins and, for the level sweep, their positions (= output ids).
+// Output fanins and either the output levels (olevel non-null: every
+// survivor's level is its wave) or, for the level sweep, the fanin positions
+// (= output ids).
 __global__ void k_rw_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ f0,
                           const uint32_t *__restrict__ f1, const uint32_t *__restrict__ rep,
                           const uint32_t *__restrict__ newId, uint32_t *of0, uint32_t *of1, uint32_t *fa,
-                          uint32_t *fb, const uint32_t *__restrict__ tagIn, uint32_t *tagOut) {
+                          uint32_t *fb, const uint32_t *__restrict__ wave, uint32_t *olevel,
+                          const uint32_t *__restrict__ tagIn, uint32_t *tagOut) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = tid; i <= nPIs; i += stride) { fa[i] = NOFAN; fb[i] = NOFAN; }
+    for (int64_t i = tid; i <= nPIs; i += stride) {
+        if (olevel) olevel[i] = 0u;
+        else { fa[i] = NOFAN; fb[i] = NOFAN; }
+    }
     for (int64_t p = tid; p < m; p += stride) {
         const uint32_t v = sorted[p];
         uint32_t a = out_lit(f0[v], nPIs, rep, newId), b = out_lit(f1[v], nPIs, rep, newId);
@@ -1219,8 +1243,12 @@ __global__ void k_rw_emit(int32_t m, int32_t nPIs, const uint32_t *__restrict__ 
         const int64_t o = nPIs + 1 + p;
         of0[o] = a;
         of1[o] = b;
-        fa[o] = LID(a);
-        fb[o] = LID(b);
+        if (olevel) {
+            olevel[o] = wave[v];
+        } else {
+            fa[o] = LID(a);
+            fb[o] = LID(b);
+        }
         if (tagOut) tagOut[o] = tagIn[v];
     }
 }
@@ -1513,7 +1541,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         uint32_t *survIds = t.get<uint32_t>(N), *survWave = t.get<uint32_t>(N);
         uint32_t *sortIds = t.get<uint32_t>(N), *sortWave = t.get<uint32_t>(N), *newId = t.get<uint32_t>(N);
         uint32_t *d_misc = t.get<uint32_t>(16);   // [0] err, [1] max level, [2] dirty count, [3] changed,
-                                                  // [4] survivors, [6..7] probes
+                                                  // [4] survivors, [6..7] probes, [8] a path got shorter
         SweepCtl *ctl = t.get<SweepCtl>(1);
         if (!ref || !order || !dead || !surv || !posOf || !fa || !fb || !levPos || !wave || !rep || !inD || !dlist ||
             !survIds || !survWave || !sortIds || !sortWave || !newId || !d_misc || !ctl)
@@ -1612,6 +1640,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
             if (misc[3] == 0 && nD == prevD) break;
         }
         // 4. survivors in (wave, id) order
+        k_rw_shorter<<<gN, kBlock, 0, st>>>(N, nPIs, dead, rep, wave, &d_misc[8]);
         k_rw_survivors<<<gN, kBlock, 0, st>>>(N, nPIs, dead, rep, surv);
         size_t tb = 0;
         rocprim::counting_iterator<uint32_t> ids(0u);
@@ -1646,24 +1675,31 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         GLS_HIP(ctx, hipMemsetAsync(out.f0, 0, sizeof(uint32_t) * (nPIs + 1), st));
         GLS_HIP(ctx, hipMemsetAsync(out.f1, 0, sizeof(uint32_t) * (nPIs + 1), st));
         k_new_ids<<<grid_for(m + nPIs + 1), kBlock, 0, st>>>(N, nPIs, (int32_t)m, sorted, newId);
-        // fa / fb are free again: they take the output's fanin ids
+        // 5. output levels: the survivors' waves when no representative lies on
+        // a lower wave than its node, else a level sweep over the output (fa /
+        // fb are free again: they take the output's fanin ids)
+        const bool sweepOut = misc[8] != 0u;
         k_rw_emit<<<grid_for(nNew), kBlock, 0, st>>>((int32_t)m, nPIs, sorted, in.f0, in.f1, rep, newId, out.f0,
-                                                     out.f1, fa, fb, tagIn, tags);
+                                                     out.f1, fa, fb, wave, sweepOut ? nullptr : out.level, tagIn,
+                                                     tags);
         if (nPOs > 0) k_rw_pos_out<<<grid_for(nPOs), kBlock, 0, st>>>(nPOs, nPIs, in.pos, rep, newId, out.pos);
-        GLS_HIP(ctx, hipMemsetAsync(&d_misc[0], 0, sizeof(uint32_t) * 2, st));
-        k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)nNew, fa, fb, out.level, &d_misc[0], &d_misc[1]);
         GLS_HIP(ctx, hipGetLastError());
-        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
-        if (misc[0]) {
-            dev_graph_free(ctx, out);
-            if (tags) { pool_put(ctx, tags); *tagOut = nullptr; }
-            return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: output not topological");
+        if (sweepOut) {
+            GLS_HIP(ctx, hipMemsetAsync(&d_misc[0], 0, sizeof(uint32_t) * 2, st));
+            k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)nNew, fa, fb, out.level, &d_misc[0], &d_misc[1]);
+            GLS_HIP(ctx, hipGetLastError());
+            GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+            if (misc[0]) {
+                dev_graph_free(ctx, out);
+                if (tags) { pool_put(ctx, tags); *tagOut = nullptr; }
+                return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: output not topological");
+            }
         }
         if (depthOut) *depthOut = depth;
         if (info) {
             info->levels_enumerated = depth;
             info->hash_probes += (int64_t)((uint64_t)misc[6] | (uint64_t)misc[7] << 32);
-            info->final_strash = 1;
+            info->final_strash = sweepOut ? 1 : 3;
             info->strash_rounds = (int64_t)r + 1;
         }
     }

@@ -67,8 +67,10 @@ typedef struct {
      * could not run reports the schedule that replaced it), and the final
      * strash: 0 = the wave sweep over the composed graph, 1 = incremental
      * (only what the replacements changed is rehashed), 2 = incremental that
-     * fell back to the wave sweep; strash_rounds = rounds of the incremental
-     * strash */
+     * fell back to the wave sweep, 3 = incremental whose output levels are
+     * the composed graph's waves (no merge or fold put a node's
+     * representative on a lower wave, so no level sweep over the output);
+     * strash_rounds = rounds of the incremental strash */
     int32_t cut_schedule, final_strash;
     int64_t strash_rounds;
     /* gls_rewrite with GLS_RW_GRAPH_CUTS: host time of capturing the cut
