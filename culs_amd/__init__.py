@@ -54,7 +54,7 @@ class OpInfo(ctypes.Structure):
                 ("kernel_sample_nodes", ctypes.c_int64), ("hash_probes", ctypes.c_int64),
                 ("cut_schedule", ctypes.c_int32), ("final_strash", ctypes.c_int32),
                 ("strash_rounds", ctypes.c_int64), ("cut_graph_ms", ctypes.c_double),
-                ("cut_graph_nodes", ctypes.c_int64)]
+                ("cut_graph_nodes", ctypes.c_int64), ("strash_sweeps", ctypes.c_int32), ("pad0", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

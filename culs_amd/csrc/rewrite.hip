@@ -1955,6 +1955,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
             info->final_strash = stInfo.final_strash;
             info->cut_schedule = cutRan;
             info->strash_rounds = stInfo.strash_rounds;
+            info->strash_sweeps = stInfo.strash_sweeps;
             info->cut_graph_ms = ctx->cut_graph_ms;
             info->cut_graph_nodes = ctx->cut_graph_nodes;
             double kms = 0;

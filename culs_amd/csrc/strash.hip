@@ -1023,6 +1023,35 @@ __global__ void k_rw_key_relax(int32_t N, int32_t nPIs, const uint32_t *__restri
     if (mv) atomicAdd(moved, mv);
 }
 
+// Sets *loose if some live AND's key is not one more than the larger key of
+// its fanins, and raises *maxKey to the largest live key.  If no key is loose
+// the keys are the composed graph's levels: the PIs and the constant have key
+// 0 (their input level), and 0 there with 1 + the larger fanin value at every
+// live AND defines the longest-path level uniquely.
+__global__ void k_rw_key_loose(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
+                               const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
+                               const uint32_t *__restrict__ key, uint32_t *loose, uint32_t *maxKey) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    bool any = false;
+    uint32_t mx = 0;
+    for (int64_t v = nPIs + 1 + tid; v < N; v += stride) {
+        if (dead[v]) continue;
+        const uint32_t ka = key[LID(f0[v])], kb = key[LID(f1[v])], k = key[v];
+        any |= k != 1u + (ka > kb ? ka : kb);
+        mx = k > mx ? k : mx;
+    }
+    if (__ballot(any) != 0 && (threadIdx.x & 63) == 0) atomicOr(loose, 1u);
+    if (mx) atomicMax(maxKey, mx);
+}
+
+// The level of every position when the keys are the levels: 0 for the PIs,
+// the constant and dangling nodes.
+__global__ void k_rw_levpos(int32_t N, int32_t nPIs, const uint8_t *__restrict__ dead,
+                            const uint32_t *__restrict__ key, const uint32_t *__restrict__ posOf, uint32_t *levPos) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) levPos[posOf[v]] = (v <= nPIs || dead[v]) ? 0u : key[v];
+}
+
 __global__ void k_iota_u32(int32_t N, uint32_t *x) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = tid; v < N; v += stride) x[v] = (uint32_t)v;
@@ -1541,7 +1570,8 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         uint32_t *survIds = t.get<uint32_t>(N), *survWave = t.get<uint32_t>(N);
         uint32_t *sortIds = t.get<uint32_t>(N), *sortWave = t.get<uint32_t>(N), *newId = t.get<uint32_t>(N);
         uint32_t *d_misc = t.get<uint32_t>(16);   // [0] err, [1] max level, [2] dirty count, [3] changed,
-                                                  // [4] survivors, [6..7] probes, [8] a path got shorter
+                                                  // [4] survivors, [6..7] probes, [8] a path got shorter,
+                                                  // [9] a key is loose, [10] largest key
         SweepCtl *ctl = t.get<SweepCtl>(1);
         if (!ref || !order || !dead || !surv || !posOf || !fa || !fb || !levPos || !wave || !rep || !inD || !dlist ||
             !survIds || !survWave || !sortIds || !sortWave || !newId || !d_misc || !ctl)
@@ -1588,15 +1618,27 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
             GLS_HIP(ctx, rocprim::radix_sort_pairs(tmpk, tbk, fa, fb, levPos, order, (size_t)N, 0u, 32u, st));
         }
         k_rw_pos<<<gN, kBlock, 0, st>>>(N, order, posOf);
-        k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
+        // the waves: the keys themselves where every live key is tight, else a
+        // level sweep over the positions
+        k_rw_key_loose<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, fa, &d_misc[9], &d_misc[10]);
         GLS_HIP(ctx, hipGetLastError());
         uint32_t misc[16] = {};
         GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
-        if (misc[0]) return fallback("positions");
-        k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)N, fa, fb, levPos, &d_misc[0], &d_misc[1]);
-        GLS_HIP(ctx, hipGetLastError());
-        GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
-        if (misc[0]) return fallback("sweep order");
+        const bool sweepIn = misc[9] != 0u;
+        if (sweepIn) {
+            k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
+            GLS_HIP(ctx, hipGetLastError());
+            GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+            if (misc[0]) return fallback("positions");
+            k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)N, fa, fb, levPos, &d_misc[0], &d_misc[1]);
+            GLS_HIP(ctx, hipGetLastError());
+            GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+            if (misc[0]) return fallback("sweep order");
+        } else {
+            k_rw_levpos<<<gN, kBlock, 0, st>>>(N, nPIs, dead, fa, posOf, levPos);
+            GLS_HIP(ctx, hipGetLastError());
+            misc[1] = misc[10];
+        }
         const int32_t depth = (int32_t)misc[1];
         k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD,
                                          dlist, &d_misc[2]);
@@ -1699,8 +1741,9 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         if (info) {
             info->levels_enumerated = depth;
             info->hash_probes += (int64_t)((uint64_t)misc[6] | (uint64_t)misc[7] << 32);
-            info->final_strash = sweepOut ? 1 : 3;
+            info->final_strash = 1;
             info->strash_rounds = (int64_t)r + 1;
+            info->strash_sweeps = (sweepIn ? 1 : 0) + (sweepOut ? 1 : 0);
         }
     }
     return GLS_OK;

@@ -67,10 +67,8 @@ typedef struct {
      * could not run reports the schedule that replaced it), and the final
      * strash: 0 = the wave sweep over the composed graph, 1 = incremental
      * (only what the replacements changed is rehashed), 2 = incremental that
-     * fell back to the wave sweep, 3 = incremental whose output levels are
-     * the composed graph's waves (no merge or fold put a node's
-     * representative on a lower wave, so no level sweep over the output);
-     * strash_rounds = rounds of the incremental strash */
+     * fell back to the wave sweep; strash_rounds = rounds of the incremental
+     * strash */
     int32_t cut_schedule, final_strash;
     int64_t strash_rounds;
     /* gls_rewrite with GLS_RW_GRAPH_CUTS: host time of capturing the cut
@@ -78,6 +76,12 @@ typedef struct {
      * of that graph (cuts_ms covers capture, instantiation and execution) */
     double cut_graph_ms;
     int64_t cut_graph_nodes;
+    /* gls_rewrite's incremental final strash: the one-workgroup level sweeps
+     * it ran (0-2).  The composed graph's levels are swept only where its
+     * relaxed topological keys are not already its levels, and the output's
+     * only where a merge or fold put a node's representative on a lower
+     * level than the node. */
+    int32_t strash_sweeps, pad0;
 } gls_op_info;
 
 /* ---- context ---------------------------------------------------------- */
