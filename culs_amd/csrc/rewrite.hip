@@ -378,21 +378,21 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     m.leaves[0] = m.leaves[1] = m.leaves[2] = m.leaves[3] = 0;
     m.nLeaves = 0; m.tt = 0; m.value = 0; m.used = 0;
     mm = m;
-    if (a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4) {
+    {
+        // every lane runs the whole computation (no divergent branches; a
+        // failed merge leaves m as initialised, so every leaf read below is a
+        // valid id) and the status says which results the replay may use
+        const bool cand = a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4;
         int phA, phB;
         const int ok = merge_ordered(a, b, m, phA, phB);
-        if (!ok) {
-            status = C_FAIL;
-        } else {
-            status = C_OK;
-            uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
-            t0 = tt_stretch(t0, a.nLeaves, phA);
-            t1 = tt_stretch(t1, b.nLeaves, phB);
-            m.tt = (t0 & t1) & 0xFFFF;
-            mm = m;
-            shrunk = minimize_support(mm);
-            mm.value = cut_value(mm, nRef);
-        }
+        uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
+        t0 = tt_stretch(t0, a.nLeaves, phA);
+        t1 = tt_stretch(t1, b.nLeaves, phB);
+        m.tt = (t0 & t1) & 0xFFFF;
+        mm = m;
+        shrunk = minimize_support(mm);
+        mm.value = cut_value(mm, nRef);
+        status = !cand ? C_SKIP : !ok ? C_FAIL : C_OK;
     }
     // per-candidate words read by the replay: status, shrink flag, leaf counts
     // and value in one; both truth tables in one; both leaf signatures
