@@ -60,6 +60,18 @@ class OpInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("cut_wide", ctypes.c_int32), ("rw_sweep_width", ctypes.c_int32),
+                ("strash_table_factor", ctypes.c_int32), ("strash_wave_batch", ctypes.c_int32),
+                ("balance_narrow_max", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+GLS_TUNE_RW, GLS_TUNE_RWZ, GLS_TUNE_STRASH, GLS_TUNE_BALANCE = 0, 1, 2, 3
+
+
 class CecInfo(ctypes.Structure):
     _fields_ = [("equivalent", ctypes.c_int32), ("exhaustive", ctypes.c_int32), ("po", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("pattern", ctypes.c_int64), ("patterns", ctypes.c_int64)]
@@ -102,6 +114,11 @@ _SIGS = {
     "gls_batch_strash": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_rewrite": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_balance": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "gls_ctx_get_tuning": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_void_p]),
+    "gls_ctx_set_tuning": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_void_p]),
+    "gls_autotune": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
     "gls_reorder": (ctypes.c_int, [_P, _P]),
     "gls_enumerate_cuts": (ctypes.c_int, [_P, _P]),
     "gls_evaluate": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
@@ -221,6 +238,28 @@ class Context:
         self.check(lib().gls_batch_balance(self._h, self._handles(aigs), len(aigs), 1, 1 if update_level else 0),
                    "gls_batch_balance")
         return aigs
+
+    def get_tuning(self, objects):
+        """The pass settings used for graphs (or batch arenas) of `objects` objects."""
+        t = Tuning()
+        self.check(lib().gls_ctx_get_tuning(self._h, objects, ctypes.byref(t)), "gls_ctx_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, objects, **settings):
+        """Changes some settings for that size bucket (objects < 0: every bucket)."""
+        t = Tuning(**self.get_tuning(max(objects, 1)))
+        for k, v in settings.items():
+            setattr(t, k, v)
+        self.check(lib().gls_ctx_set_tuning(self._h, objects, ctypes.byref(t)), "gls_ctx_set_tuning")
+
+    def autotune(self, aigs, command=GLS_TUNE_RW, reps=2):
+        """Times `command` on device copies of the AIGs under candidate settings
+        and keeps the fastest for their size bucket -> (settings, best_ms, start_ms)."""
+        t = Tuning()
+        best, start = ctypes.c_double(0), ctypes.c_double(0)
+        self.check(lib().gls_autotune(self._h, self._handles(aigs), len(aigs), command, reps, ctypes.byref(t),
+                                      ctypes.byref(best), ctypes.byref(start)), "gls_autotune")
+        return t.as_dict(), best.value, start.value
 
     def gen_multiplier(self, nbits, redundant, seed):
         h = _P()

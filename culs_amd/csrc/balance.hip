@@ -24,13 +24,14 @@
 // Literal values decide ties in the input order; there the reference's own
 // values are used (lit 0 = TRUE, common.h:285-286), not the AIGER ones.
 //
-// Mapping to MI355X: cover discovery is a frontier sweep (one launch per
-// level, each node touched once per level it appears at).  Reconstruction of
+// Mapping to MI355X: cover discovery is two frontier sweeps (reachability,
+// then a topological sweep of the cover graph; each root is touched twice,
+// several rounds per host check).  Reconstruction of
 // narrow levels runs in ONE persistent workgroup across levels and steps, so
 // a step costs a few workgroup barriers instead of kernel launches and a host
 // round trip; the hash tables are global and all of their traffic is atomics,
-// which the L2 serves.  Wide levels (more than BAL_WG_MAX covers) run each
-// phase of a step as a grid-wide kernel.
+// which the L2 serves.  Wide levels (more than balance_narrow_max covers, a
+// gls_tuning setting) run each phase of a step as a grid-wide kernel.
 #include "gls_internal.h"
 #include "dev_common.h"
 #include <rocprim/device/device_radix_sort.hpp>
@@ -48,7 +49,6 @@ constexpr int BAL_STACK = 128;        // DFS_COVER_STACK_SIZE, balance.cu:18
 constexpr int BAL_MAX_COVER = 1024;   // MAX_LOCAL_COVER_SIZE, balance.cu:17
 constexpr int BAL_BATCH = 32768;      // iterLen, balance.cu:1129
 constexpr int BAL_WG = 1024;          // threads of the persistent reconstruction workgroup
-constexpr int BAL_WG_MAX = 4096;      // widest level handled by it
 constexpr int COVER_THREADS = 16384;  // threads of a cover-discovery launch (each owns COVER_SCRATCH slots),
 constexpr int COVER_THREADS_MAX = 131072;   // scaled up to this with the graph (a level's frontier in one round)
 // the size limit is checked before an expansion, so a cover can end up to a
@@ -423,7 +423,8 @@ __device__ __forceinline__ void rec_record(const RecArgs &r, uint32_t c, uint32_
 }
 
 // The persistent workgroup: levels [lFirst .. lLast] (descending), every one
-// at most BAL_WG_MAX covers wide; starts[l] = first index of level l in roots.
+// at most balance_narrow_max covers wide; starts[l] = first index of level l
+// in roots.
 __global__ void __launch_bounds__(BAL_WG) k_rec_narrow(RecArgs r, const uint32_t *__restrict__ starts, int32_t lFirst,
                                                        int32_t lLast) {
     __shared__ uint32_t s_any, s_max, s_counter;
@@ -672,6 +673,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
                 uint32_t **tagOut, std::vector<uint32_t> *splitTags) {
     const bool tagged = tagIn && tagOut;
     const int32_t n = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs;
+    const uint32_t narrowMax = (uint32_t)tuning_for(ctx, n).balance_narrow_max;
     hipStream_t st = ctx->stream;
     Tmp t{ctx};
     const auto t0 = std::chrono::steady_clock::now();
@@ -809,7 +811,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     uint32_t nBatches = 0;
     for (uint32_t l = 0; l < levels; l++) {
         const uint32_t cnt = starts[l + 1] - starts[l];
-        if (cnt <= (uint32_t)BAL_WG_MAX) continue;
+        if (cnt <= narrowMax) continue;
         batchOff[l] = nBatches;
         nBatches += (cnt + std::min(cnt, batchLimit) - 1) / std::min(cnt, batchLimit);
     }
@@ -866,7 +868,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     while (l >= 0) {
         // a run of narrow levels goes to the persistent workgroup
         int32_t e = l;
-        while (e >= 0 && starts[e + 1] - starts[e] <= (uint32_t)BAL_WG_MAX) e--;
+        while (e >= 0 && starts[e + 1] - starts[e] <= narrowMax) e--;
         if (e < l) {
             k_rec_narrow<<<1, BAL_WG, 0, st>>>(ra, d_starts, l, e + 1);
             GLS_HIP(ctx, hipGetLastError());

@@ -365,19 +365,33 @@ int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t u
 
 } // namespace
 
+// objects of the batch arena (for the size bucket of its settings)
+static int64_t batch_objects(gls_aig **aigs, int32_t count) {
+    int64_t n = 1;
+    for (int32_t k = 0; aigs && k < count; k++)
+        if (aigs[k]) n += aigs[k]->nObjs;
+    return n;
+}
+
 extern "C" int gls_batch_strash(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t flags) {
+    if (!ctx) return GLS_ERR_ARG;
     if (flags != 0) return gls_fail(ctx, GLS_ERR_ARG, "gls_batch_strash: unknown flags 0x%x", flags);
+    TuneScope tune(ctx, batch_objects(aigs, count));
     return batch_pass(ctx, aigs, count, Pass::Strash, 0, 0);
 }
 
 extern "C" int gls_batch_rewrite(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t use_zeros, int32_t flags) {
+    if (!ctx) return GLS_ERR_ARG;
+    TuneScope tune(ctx, batch_objects(aigs, count));
     return batch_pass(ctx, aigs, count, Pass::Rewrite, use_zeros ? 1 : 0, flags);
 }
 
 extern "C" int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_dec_id,
                                  int32_t update_level) {
+    if (!ctx) return GLS_ERR_ARG;
     if (sort_dec_id != 1)
         return gls_fail(ctx, GLS_ERR_ARG, "batch balance: only sort_dec_id = 1 (level, then literal) is supported");
+    TuneScope tune(ctx, batch_objects(aigs, count));
     GLS_TRY(batch_pass(ctx, aigs, count, Pass::Balance, 0, 0));
     if (update_level) GLS_TRY(batch_pass(ctx, aigs, count, Pass::Strash, 0, 0));
     return GLS_OK;

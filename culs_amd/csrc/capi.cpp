@@ -107,6 +107,7 @@ extern "C" int gls_ctx_create(int device, gls_ctx **out) {
     if (!out) return GLS_ERR_ARG;
     *out = nullptr;
     gls_ctx *ctx = new gls_ctx();
+    for (auto &t : ctx->tune) t = kTuningDefault;
     if (device >= 0) {
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count <= device) {
@@ -446,6 +447,7 @@ extern "C" int gls_strash(gls_aig *a, int32_t flags) {
     if (flags & ~GLS_ST_CPU) return gls_fail(a->ctx, GLS_ERR_ARG, "gls_strash: unknown flags 0x%x", flags);
     if (flags & GLS_ST_CPU) return strash_cpu(a);
     gls_ctx *ctx = a->ctx;
+    TuneScope tune(ctx, a->nObjs);
     GLS_TRY(aig_ensure_device(a));
     GLS_HIP(ctx, hipEventRecord(ctx->ev[14], ctx->stream));
     DevGraph out;
@@ -466,6 +468,7 @@ extern "C" int gls_strash(gls_aig *a, int32_t flags) {
 extern "C" int gls_rewrite(gls_aig *a, int32_t use_zeros, int32_t flags) {
     if (!a) return GLS_ERR_ARG;
     gls_ctx *ctx = a->ctx;
+    TuneScope tune(ctx, a->nObjs);
     GLS_TRY(aig_ensure_device(a));
     if (!a->strashed || !a->dev_level_valid) GLS_TRY(gls_strash(a, 0));
     GLS_HIP(ctx, hipEventRecord(ctx->ev[14], ctx->stream));
@@ -484,6 +487,7 @@ extern "C" int gls_balance(gls_aig *a, int32_t sort_dec_id, int32_t update_level
     gls_ctx *ctx = a->ctx;
     if (sort_dec_id != 1)
         return gls_fail(ctx, GLS_ERR_ARG, "balance: only sort_dec_id = 1 (level, then literal) is supported");
+    TuneScope tune(ctx, a->nObjs);
     GLS_TRY(aig_ensure_device(a));
     DevGraph out;
     gls_op_info info = {};
@@ -509,6 +513,7 @@ extern "C" int gls_aig_last_op(gls_aig *a, gls_op_info *out) {
 extern "C" int gls_reorder(gls_aig *a, uint32_t *level_out) {
     if (!a) return GLS_ERR_ARG;
     gls_ctx *ctx = a->ctx;
+    TuneScope tune(ctx, a->nObjs);
     GLS_TRY(aig_ensure_device(a));
     if (!a->strashed || !a->dev_level_valid) GLS_TRY(gls_strash(a, 0));
     DevGraph out;

@@ -64,6 +64,27 @@ struct gls_ctx {
     // host time of the last cut-graph capture + instantiation, and its launches
     double cut_graph_ms = 0;
     int64_t cut_graph_nodes = 0;
+    // settings per size bucket (floor(log2(objects))); `active` is the set an
+    // entry point chose for the pass it runs (TuneScope), else null
+    gls_tuning tune[64];
+    const gls_tuning *active = nullptr;
+};
+
+// the defaults of gls_tuning
+constexpr gls_tuning kTuningDefault{2048, 1024, 4, 4, 4096, 0};
+int tuning_bucket(int64_t objects);
+// the settings of the running pass (the active set, or the bucket of `objects`)
+inline const gls_tuning &tuning_for(const gls_ctx *ctx, int64_t objects) {
+    return ctx->active ? *ctx->active : ctx->tune[tuning_bucket(objects)];
+}
+// an entry point's pass uses the settings of its input size throughout
+struct TuneScope {
+    gls_ctx *ctx;
+    const gls_tuning *prev;
+    TuneScope(gls_ctx *c, int64_t objects) : ctx(c), prev(c->active) {
+        if (!prev) c->active = &c->tune[tuning_bucket(objects)];
+    }
+    ~TuneScope() { ctx->active = prev; }
 };
 
 // A device-resident AIG in the AIGER literal convention.

@@ -1492,9 +1492,6 @@ int dev_reorder(gls_ctx *ctx, const DevGraph &in, DevGraph &out, std::vector<int
 // (GLS_KERNEL_SAMPLE=0 in the environment turns this off, e.g. under a
 // profiler that collects counters per dispatch)
 static constexpr int KSAMPLE_EVERY = 64;
-// widest level (nodes) for which cut enumeration recomputes fanin cones to
-// cover several levels per launch
-static constexpr int32_t CUT_WIDE = 2048;
 static constexpr int KSAMPLE_WIN = 8;
 static const bool kSampleLaunches = [] {
     const char *e = getenv("GLS_KERNEL_SAMPLE");
@@ -1630,10 +1627,11 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     } else {
         // default: three levels per launch (k_cuts_triple); GLS_RW_PAIR_CUTS:
         // two (k_cuts_pair); GLS_RW_LEVEL_CUTS: one (k_cuts_level).  A group
-        // with a level wider than CUT_WIDE nodes fills the GPU by itself: the
+        // with a level wider than cut_wide nodes fills the GPU by itself: the
         // recomputed cones would only add work there, so its levels get one
         // launch each.
         const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : 3;
+        const int32_t cutWide = tuning_for(ctx, n).cut_wide;
         ran = step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS : 0;
         struct CutLaunch { int kind; int32_t s, m, m2, e; };
         std::vector<CutLaunch> plan;
@@ -1643,7 +1641,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
             const int32_t m2 = levelStart[std::min(l + 2, L)];
             const int32_t e = levelStart[std::min(l + step, L)];
             if (e <= s) continue;
-            if (step > 1 && std::max(std::max(m - s, m2 - m), e - m2) > CUT_WIDE) {
+            if (step > 1 && std::max(std::max(m - s, m2 - m), e - m2) > cutWide) {
                 for (int j = l; j < std::min(l + step, L); j++)
                     if (levelStart[j + 1] > levelStart[j])
                         plan.push_back({1, levelStart[j], levelStart[j + 1], levelStart[j + 1], levelStart[j + 1]});
@@ -1715,7 +1713,7 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
             if (ec != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cut graph capture: %s", hipGetErrorString(ec));
             const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             if (ei != hipSuccess) {
-                hipGraphDestroy(graph);
+                (void)hipGraphDestroy(graph);
                 return gls_fail(ctx, GLS_ERR_HIP, "cut graph instantiation: %s", hipGetErrorString(ei));
             }
             ctx->cut_graph_ms =
@@ -1724,8 +1722,8 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
             hipError_t el = hipGraphLaunch(exec, st);
             // the executable graph is released once its launch has run
             if (el == hipSuccess) el = hipStreamSynchronize(st);
-            hipGraphExecDestroy(exec);
-            hipGraphDestroy(graph);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
             if (el != hipSuccess) return gls_fail(ctx, GLS_ERR_HIP, "cut graph launch: %s", hipGetErrorString(el));
             ran |= GLS_RW_GRAPH_CUTS;
         }

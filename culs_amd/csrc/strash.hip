@@ -911,9 +911,6 @@ constexpr uint32_t SW_NOLEV = 0xFFFFFFFFu;
 constexpr uint32_t SW_SPIN_LIMIT = 1u << 22;
 // rounds of the incremental strash before it gives way to the wave sweep
 constexpr uint32_t kRwRoundLimit = 4096;
-// objects per level of the rewrite's input above which the final strash is
-// the wave sweep (see dev_strash_rw)
-constexpr int64_t kRwSweepWidth = 1024;
 
 __device__ __forceinline__ uint32_t sw_lds_load(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1366,8 +1363,9 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     NodeOfl *ofls = t.get<NodeOfl>(n);
     SweepCtl *ctl = t.get<SweepCtl>(1);
     // room for every live node's own key plus a changed key per node
+    const gls_tuning &tune = tuning_for(ctx, n);
     uint64_t cap = 1024;
-    while (cap < 4ull * (uint64_t)n) cap <<= 1;
+    while (cap < (uint64_t)tune.strash_table_factor * (uint64_t)n) cap <<= 1;
     if (cap > (1ull << 31)) return gls_fail(ctx, GLS_ERR_ARG, "strash: %d objects exceed the table's slot range", n);
     unsigned long long *ht = t.get<unsigned long long>(2 * cap);
     if (!ref || !order || !pending || !cnt || !adjOff || !cursor || !adj || !rl || !wave ||
@@ -1445,7 +1443,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
             waves += kn - k;
             k = kn;
         } else {
-            for (int b = 0; b < 4; b++, k++, waves++)
+            for (int b = 0; b < tune.strash_wave_batch; b++, k++, waves++)
                 k_wave_multi<<<sweep_grid(cur), kBlock, 0, st>>>(k, a);
             GLS_HIP(ctx, hipGetLastError());
         }
@@ -1555,7 +1553,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
     // the two of them; the wave sweep costs about 2.6 us per level plus its
     // per-node work.  On wide graphs (many objects per level, as in a batched
     // arena) the wave sweep is the faster one.
-    if ((int64_t)N > kRwSweepWidth * (int64_t)std::max(d.depth, 1)) {
+    if ((int64_t)N > (int64_t)tuning_for(ctx, d.n).rw_sweep_width * (int64_t)std::max(d.depth, 1)) {
         const int rc = dev_strash(ctx, in, out, depthOut, info, tagIn, tagOut, tagDepth);
         if (info) info->final_strash = 0;
         return rc;

@@ -179,6 +179,45 @@ int gls_batch_rewrite(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t use_z
  * more than 32768 covers on one level is balanced alone (same result). */
 int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_dec_id, int32_t update_level);
 
+/* ---- tuning ----------------------------------------------------------- */
+/* No counterpart in the reference (its launch shapes and table sizes are
+ * constants).  Settings that change only how a pass is scheduled or how much
+ * room its tables have; every setting gives the same results.  A context
+ * keeps one set per size bucket (floor(log2(objects)) of the graph or batch
+ * arena a pass runs on), initially the defaults below.
+ *   cut_wide: a group of three levels whose widest level has more nodes runs
+ *     one cut launch per level (default 2048);
+ *   rw_sweep_width: rw finishes with the incremental strash while the input
+ *     has at most this many objects per level on average, else with the
+ *     wave sweep (default 1024);
+ *   strash_table_factor: the strash table has at least this many slots per
+ *     object, a power of two, 4 or 8 (default 4);
+ *   strash_wave_batch: wide strash waves launched per host check, 1-64
+ *     (default 4);
+ *   balance_narrow_max: balance levels of at most this many covers are
+ *     rebuilt in one persistent workgroup, wider ones by grid-wide phase
+ *     kernels (default 4096). */
+typedef struct {
+    int32_t cut_wide, rw_sweep_width, strash_table_factor, strash_wave_batch, balance_narrow_max, pad;
+} gls_tuning;
+/* The settings a pass over `objects` objects uses. */
+int gls_ctx_get_tuning(gls_ctx *ctx, int64_t objects, gls_tuning *out);
+/* Sets them for that size bucket (objects < 0: every bucket); GLS_ERR_ARG
+ * for a value out of range. */
+int gls_ctx_set_tuning(gls_ctx *ctx, int64_t objects, const gls_tuning *t);
+/* Times `command` (GLS_TUNE_*) on device copies of the AIGs (one AIG: the
+ * single pass; several: the batched pass) under candidate settings, one
+ * setting at a time from the current ones (coordinate descent, `reps` runs
+ * each, the fastest kept), stores the best for the bucket of the inputs'
+ * total objects and returns it with its time and the time of the settings it
+ * started from.  The inputs are not changed. */
+#define GLS_TUNE_RW 0
+#define GLS_TUNE_RWZ 1
+#define GLS_TUNE_STRASH 2
+#define GLS_TUNE_BALANCE 3
+int gls_autotune(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t command, int32_t reps, gls_tuning *out,
+                 double *best_ms, double *start_ms);
+
 /* ---- serving ---------------------------------------------------------- */
 /* No counterpart in the reference (one script, one AIG, one process).  A
  * server owns a worker thread and its own context on `device`.  Jobs (an AIG
