@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--no-loop", action="store_true",
                     help="batch: time only the batched pass (no one-at-a-time loop, no identity check); "
                          "for counter runs")
+    ap.add_argument("--no-tune", action="store_true", help="batch: time the pass with the default settings")
     ap.add_argument("--batch-check", type=int, default=256,
                     help="members compared array by array with their one-at-a-time result")
     a = ap.parse_args()
@@ -305,7 +306,7 @@ def main():
             x.free()
         b = bench_batch(a)
         res["batch"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "loop", "speedup_vs_loop",
-                                          "identical_to_alone", "config", "stages_ms")}
+                                          "identical_to_alone", "config", "stages_ms", "tuning")}
         log("batch: %s" % res["batch"])
         v = bench_serve(a)
         res["serve"] = {k: v[k] for k in ("metric", "value", "unit", "served", "loop", "throughput_vs_loop",
@@ -416,6 +417,17 @@ def bench_batch(a):
     n_in = sum(x.n_ands for x in aigs)
     depth = max(x.stats().levels for x in aigs)
     log("strashed: %d ANDs in total, deepest %d levels" % (n_in, depth))
+    tuning = None
+    if not a.no_tune:
+        # the batched pass's settings for this arena size, picked by timing it
+        t_tune = time.time()
+        settings, tuned_ms, default_ms = ctx.autotune(aigs, culs_amd.GLS_TUNE_RW, reps=1)
+        tuning = {"settings": settings, "start_ms": round(default_ms, 3), "tuned_ms": round(tuned_ms, 3),
+                  "search_s": round(time.time() - t_tune, 1),
+                  "method": "gls_autotune: coordinate descent over the pass settings, one timed batched rw "
+                            "on device copies per candidate; the timed steps below run with the result"}
+        log("autotune: %s, %.1f ms (started from %.1f ms) in %.1fs" % (settings, tuned_ms, default_ms,
+                                                                       time.time() - t_tune))
 
     def copies():
         return [x.copy() for x in aigs]
@@ -441,7 +453,7 @@ def bench_batch(a):
     if a.no_loop:
         return {"metric": "batched rw Mnodes/s, %d mixed AIGs" % len(aigs),
                 "value": round(n_in / t_batch / 1e6, 4), "unit": "Mnodes/s",
-                "ms_per_step": round(t_batch * 1e3, 3), "loop": None,
+                "ms_per_step": round(t_batch * 1e3, 3), "loop": None, "tuning": tuning,
                 "stages_ms": {k: round(info[k], 3) for k in ("cuts_ms", "eval_ms", "strash_ms", "total_ms")}}
     # the plain loop: the same pass, one AIG at a time (each already on the device)
     c = copies()
@@ -472,6 +484,7 @@ def bench_batch(a):
         "qor": {"and_in": n_in, "and_out": n_out},
         "stages_ms": {k: round(info[k], 3) for k in ("reorder_ms", "cuts_ms", "table_ms", "eval_ms", "select_ms",
                                                       "build_ms", "strash_ms", "total_ms")},
+        "tuning": tuning,
     }
     for x in out + c + aigs:
         x.free()
