@@ -92,15 +92,20 @@ __global__ void k_pack_pos(int64_t O, int count, int32_t P, const Member *__rest
 }
 
 // split, step 1: AND counts per member
-__global__ void k_split_count(int32_t P, int32_t N, const uint32_t *__restrict__ tag, uint32_t *cnt, uint32_t *keys,
-                              uint32_t *ids) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t u = P + 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < N; u += stride) {
+__global__ void __launch_bounds__(kBlock) k_split_count(int32_t P, int32_t N, const uint32_t *__restrict__ tag,
+                                                        uint32_t *cnt, uint32_t *keys, uint32_t *ids) {
+    __shared__ uint32_t s_bins[TAG_BINS];
+    TagBins tb{s_bins};
+    tb.init();
+    int64_t b, e;
+    tag_chunk(P + 1, N, b, e);
+    for (int64_t u = b + threadIdx.x; u < e; u += blockDim.x) {
         const uint32_t k = tag[u];
-        atomicAdd(&cnt[k], 1u);
+        tb.add(cnt, k, 1u);
         keys[u - P - 1] = k;
         ids[u - P - 1] = (uint32_t)u;
     }
+    tb.flush_add(cnt, 0xFFFFFFFFu);
 }
 
 // split, step 2: local ids (ANDs: rank inside their member, in union id order)
@@ -299,7 +304,7 @@ int batch_pass(gls_ctx *ctx, gls_aig **aigs, int32_t count, Pass pass, int32_t u
     if (!cnt || !keys || !ids || !skeys || !sids || !localId) return gls_fail(ctx, GLS_ERR_NOMEM, "batch: split");
     GLS_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(uint32_t) * count, st));
     if (Ar > 0) {
-        k_split_count<<<nblocks(Ar), kB, 0, st>>>((int32_t)P, R.nObjs, rtag, cnt, keys, ids);
+        k_split_count<<<tag_grid(Ar), kBlock, 0, st>>>((int32_t)P, R.nObjs, rtag, cnt, keys, ids);
         size_t tb = 0;
         const uint32_t eb = key_bits((uint32_t)count);
         GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, keys, skeys, ids, sids, (size_t)Ar, 0u, eb, st));

@@ -87,6 +87,50 @@ __device__ __forceinline__ void block_add_u64(unsigned long long *dst, uint32_t 
     if (threadIdx.x == 0 && s_sum) atomicAdd(dst, s_sum);
 }
 
+// Per-tag reductions (a count or a maximum per member of a batch) over a
+// range of items.  One global atomic per item on a per-member word would
+// serialize on that word (a member's items all hit it), so each workgroup
+// takes a contiguous chunk of the range, reduces tags below TAG_BINS in LDS
+// and adds its non-empty bins to the global words; larger tags go to the
+// global words directly.  Launch with tag_grid(items) workgroups of kBlock.
+constexpr int TAG_BINS = 8192;
+inline int tag_grid(int64_t items) {
+    int64_t g = (items + 4 * kBlock - 1) / (4 * kBlock);
+    return (int)(g < 1 ? 1 : g > 512 ? 512 : g);
+}
+struct TagBins {
+    uint32_t *bins;   // TAG_BINS words of LDS
+    __device__ void init() {
+        for (int i = threadIdx.x; i < TAG_BINS; i += blockDim.x) bins[i] = 0;
+        __syncthreads();
+    }
+    __device__ void add(uint32_t *global, uint32_t tag, uint32_t x) {
+        if (tag < (uint32_t)TAG_BINS) atomicAdd(&bins[tag], x);
+        else atomicAdd(&global[tag], x);
+    }
+    __device__ void max(uint32_t *global, uint32_t tag, uint32_t x) {
+        if (tag < (uint32_t)TAG_BINS) atomicMax(&bins[tag], x);
+        else atomicMax(&global[tag], x);
+    }
+    // nTags: the global words that exist (bins past it stay empty)
+    __device__ void flush_add(uint32_t *global, uint32_t nTags) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nTags && i < (uint32_t)TAG_BINS; i += blockDim.x)
+            if (bins[i]) atomicAdd(&global[i], bins[i]);
+    }
+    __device__ void flush_max(uint32_t *global, uint32_t nTags) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nTags && i < (uint32_t)TAG_BINS; i += blockDim.x)
+            if (bins[i]) atomicMax(&global[i], bins[i]);
+    }
+};
+// this workgroup's chunk [begin, end) of [first, last)
+__device__ __forceinline__ void tag_chunk(int64_t first, int64_t last, int64_t &begin, int64_t &end) {
+    const int64_t total = last - first, chunk = (total + gridDim.x - 1) / gridDim.x;
+    begin = first + (int64_t)blockIdx.x * chunk;
+    end = begin + chunk < last ? begin + chunk : last;
+}
+
 inline int grid_for(int64_t n, int cap = 4096) {
     int64_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;

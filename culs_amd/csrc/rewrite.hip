@@ -1376,14 +1376,20 @@ struct Timer {
 // the reordered numbering an AIG's PIs come first, then its ANDs level-major
 // (ties by id), exactly as its own reordered numbering, so the rank inside the
 // tag group + 1 is that id.
-__global__ void k_sig_keys(int32_t n, const uint32_t *__restrict__ tag, uint32_t *keys, uint32_t *vals,
-                           uint32_t *hist) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t u = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += stride) {
-        keys[u - 1] = tag[u];
+__global__ void __launch_bounds__(kBlock) k_sig_keys(int32_t n, const uint32_t *__restrict__ tag, uint32_t *keys,
+                                                     uint32_t *vals, uint32_t *hist) {
+    __shared__ uint32_t s_bins[TAG_BINS];
+    TagBins tb{s_bins};
+    tb.init();
+    int64_t b, e;
+    tag_chunk(1, n, b, e);
+    for (int64_t u = b + threadIdx.x; u < e; u += blockDim.x) {
+        const uint32_t k = tag[u];
+        keys[u - 1] = k;
         vals[u - 1] = (uint32_t)u;
-        atomicAdd(&hist[tag[u]], 1u);
+        tb.add(hist, k, 1u);
     }
+    tb.flush_add(hist, 0xFFFFFFFFu);
 }
 
 __global__ void k_sig_bits(int32_t m, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ svals,
@@ -1411,7 +1417,7 @@ static int sig_bits_for_tags(gls_ctx *ctx, int32_t n, const uint32_t *tag, uint8
     uint32_t *svals = t.get<uint32_t>(m), *hist = t.get<uint32_t>(maxTag + 2), *start = t.get<uint32_t>(maxTag + 2);
     if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "signature bits: allocation");
     GLS_HIP(ctx, hipMemsetAsync(hist, 0, sizeof(uint32_t) * (maxTag + 2), st));
-    k_sig_keys<<<grid_for(m), kBlock, 0, st>>>(n, tag, keys, vals, hist);
+    k_sig_keys<<<tag_grid(m), kBlock, 0, st>>>(n, tag, keys, vals, hist);
     size_t tb2 = 0;
     const uint32_t eb = bits_for(maxTag + 1);
     GLS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb2, keys, skeys, vals, svals, (size_t)m, 0u, eb, st));

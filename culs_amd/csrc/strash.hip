@@ -1335,11 +1335,16 @@ static int run_sweep(gls_ctx *ctx, SweepCtl *d_ctl, uint32_t first, F launch, Sw
     return GLS_OK;
 }
 
-__global__ void k_tag_depth(int32_t n, int32_t nPIs, const uint32_t *__restrict__ wave,
-                            const uint32_t *__restrict__ tag, uint32_t *tagDepth) {
-    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = nPIs + 1 + tid; v < n; v += stride)
-        if (wave[v]) atomicMax(&tagDepth[tag[v]], wave[v]);   // dangling nodes keep wave 0
+__global__ void __launch_bounds__(kBlock) k_tag_depth(int32_t n, int32_t nPIs, const uint32_t *__restrict__ wave,
+                                                      const uint32_t *__restrict__ tag, uint32_t *tagDepth) {
+    __shared__ uint32_t s_bins[TAG_BINS];
+    TagBins tb{s_bins};
+    tb.init();
+    int64_t b, e;
+    tag_chunk(nPIs + 1, n, b, e);
+    for (int64_t v = b + threadIdx.x; v < e; v += blockDim.x)
+        if (wave[v]) tb.max(tagDepth, tag[v], wave[v]);   // dangling nodes keep wave 0
+    tb.flush_max(tagDepth, 0xFFFFFFFFu);
 }
 
 int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOut, gls_op_info *info,
@@ -1461,7 +1466,7 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
 
     if (processed > 0)
         k_wave_of_order<<<grid_for(processed), kBlock, 0, st>>>(processed, (uint32_t)depth, fstart, order, wave);
-    if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(n, nPIs, wave, tagIn, tagDepth);
+    if (tagIn && tagDepth) k_tag_depth<<<tag_grid(n), kBlock, 0, st>>>(n, nPIs, wave, tagIn, tagDepth);
     // 4. survivors in (wave, id) order
     k_resolve_all<<<gN, kBlock, 0, st>>>(n, nPIs, dead, rl, locOf, ht, val, surv);
     GLS_HIP(ctx, hipGetLastError());
@@ -1641,7 +1646,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD,
                                          dlist, &d_misc[2]);
         GLS_HIP(ctx, hipGetLastError());
-        if (tagIn && tagDepth) k_tag_depth<<<gN, kBlock, 0, st>>>(N, nPIs, wave, tagIn, tagDepth);
+        if (tagIn && tagDepth) k_tag_depth<<<tag_grid(N), kBlock, 0, st>>>(N, nPIs, wave, tagIn, tagDepth);
         // 3. representatives of the dirty set, in rounds
         uint32_t nD = 0;
         GLS_TRY(stream_copy(ctx, &nD, &d_misc[2], sizeof(uint32_t), hipMemcpyDeviceToHost));
