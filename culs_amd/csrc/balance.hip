@@ -164,7 +164,9 @@ __global__ void k_po_frontier(int32_t nPOs, int32_t nPIs, const uint32_t *__rest
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nPOs; i += stride) {
         const uint32_t id = LID(pos[i]);
-        if ((int32_t)id > nPIs && atomicExch(&stamp[id], 1u) != 1u) front[atomicAdd(frontLen, 1u)] = id;
+        const bool add = (int32_t)id > nPIs && atomicExch(&stamp[id], 1u) != 1u;
+        const uint32_t q = wave_append(frontLen, add);
+        if (add) front[q] = id;
     }
 }
 
@@ -232,10 +234,15 @@ __global__ void k_cover_reach(CoverArgs a, uint32_t *front0, uint32_t *front1, u
     uint32_t *nextLen = &flen[round + 1];
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     uint32_t *in = a.scratch + (size_t)tid * COVER_SCRATCH;
-    for (int64_t k = tid; k < frontLen; k += stride) {
-        const uint32_t v = front[k];
-        const uint32_t len = cover_dfs(a, v, in);
-        const uint32_t off = atomicAdd(a.poolNext, len);
+    const int64_t lane = threadIdx.x & 63;
+    // the wavefront runs the same number of iterations (its first lane's
+    // index decides), so the pool reservation sees all 64 lanes
+    for (int64_t k = tid; k - lane < (int64_t)frontLen; k += stride) {
+        const bool has = k < (int64_t)frontLen;
+        const uint32_t v = has ? front[k] : 0u;
+        const uint32_t len = has ? cover_dfs(a, v, in) : 0u;
+        const uint32_t off = wave_reserve(a.poolNext, len);
+        if (!has) continue;
         if ((uint64_t)off + len > a.poolCap) {
             atomicOr(a.err, 1u);
             a.coverLen[v] = 0;
@@ -244,10 +251,13 @@ __global__ void k_cover_reach(CoverArgs a, uint32_t *front0, uint32_t *front1, u
         for (uint32_t i = 0; i < len; i++) {
             const uint32_t lit = in[i], id = lit >> 1;
             a.pool[off + i] = lit;
+            bool add = false;
             if ((int32_t)id > a.nPIs) {
                 atomicAdd(&a.indeg[id], 1u);
-                if (atomicExch(&a.stamp[id], 1u) != 1u) next[atomicAdd(nextLen, 1u)] = id;
+                add = atomicExch(&a.stamp[id], 1u) != 1u;
             }
+            const uint32_t q = wave_append(nextLen, add);
+            if (add) next[q] = id;
         }
         a.coverOff[v] = off;
         a.coverLen[v] = len;
@@ -261,8 +271,9 @@ __global__ void k_kahn_first(int32_t nPOs, int32_t nPIs, const uint32_t *__restr
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nPOs; i += stride) {
         const uint32_t id = LID(pos[i]);
-        if ((int32_t)id > nPIs && indeg[id] == 0u && atomicExch(&stamp[id], 2u) != 2u)
-            front[atomicAdd(frontLen, 1u)] = id;
+        const bool add = (int32_t)id > nPIs && indeg[id] == 0u && atomicExch(&stamp[id], 2u) != 2u;
+        const uint32_t q = wave_append(frontLen, add);
+        if (add) front[q] = id;
     }
 }
 
@@ -278,7 +289,9 @@ __global__ void k_cover_kahn(CoverArgs a, uint32_t *front0, uint32_t *front1, ui
         const uint32_t len = a.coverLen[v], off = a.coverOff[v];
         for (uint32_t i = 0; i < len; i++) {
             const uint32_t id = a.pool[off + i] >> 1;
-            if ((int32_t)id > a.nPIs && atomicSub(&a.indeg[id], 1u) == 1u) next[atomicAdd(nextLen, 1u)] = id;
+            const bool add = (int32_t)id > a.nPIs && atomicSub(&a.indeg[id], 1u) == 1u;
+            const uint32_t q = wave_append(nextLen, add);
+            if (add) next[q] = id;
         }
     }
 }
@@ -494,15 +507,25 @@ __global__ void k_rec_insert(RecArgs r, uint32_t nb) {
 
 __global__ void k_rec_retrieve(RecArgs r, uint32_t nb) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    uint32_t mx = 0;
+    uint32_t mx = 0, made = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nb; c += stride) {
         if (r.mask[c] != 1) continue;
         const uint32_t got = r_find(r.t, r.skey[c]);
-        if (got == r.sval[c]) atomicAdd(&r.counter[3], 1u);
+        made += got == r.sval[c] ? 1u : 0u;
         r.sval[c] = got;
         mx = got > mx ? got : mx;
     }
-    if (mx) atomicMax(&r.counter[1], mx);
+    // one atomic per workgroup for each counter
+    __shared__ uint32_t s_made, s_mx;
+    if (threadIdx.x == 0) { s_made = 0; s_mx = 0; }
+    __syncthreads();
+    if (made) atomicAdd(&s_made, made);
+    if (mx) atomicMax(&s_mx, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_made) atomicAdd(&r.counter[3], s_made);
+        if (s_mx) atomicMax(&r.counter[1], s_mx);
+    }
 }
 
 __global__ void k_rec_add_back(RecArgs r, uint32_t s, uint32_t nb) {
@@ -534,8 +557,9 @@ __global__ void k_created(uint64_t cap, const unsigned long long *__restrict__ r
                           unsigned long long *keys, uint32_t *ids, uint32_t *count) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < (int64_t)cap; s += stride) {
-        if (rk[s] == EMPTY64) continue;
-        const uint32_t k = atomicAdd(count, 1u);
+        const bool full = rk[s] != EMPTY64;
+        const uint32_t k = wave_append(count, full);
+        if (!full) continue;
         keys[k] = rk[s];
         ids[k] = rv[s];
     }

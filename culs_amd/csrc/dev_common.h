@@ -87,6 +87,36 @@ __device__ __forceinline__ void block_add_u64(unsigned long long *dst, uint32_t 
     if (threadIdx.x == 0 && s_sum) atomicAdd(dst, s_sum);
 }
 
+// Appending to a shared buffer with one atomic per wavefront instead of one
+// per item: the lanes that want a slot (any subset of the active lanes) take
+// consecutive slots after the leader's reservation, in lane order.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *count, bool want) {
+    const uint64_t m = __ballot(want);
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+    uint32_t base = 0;
+    if (m && lane == (uint32_t)leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// n entries for every lane with one atomic per wavefront (n may be 0); all 64
+// lanes must be active
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *count, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t x = n;   // inclusive prefix sum over the lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)x, 63);
+    uint32_t base = 0;
+    if (lane == 63u && total) base = atomicAdd(count, total);
+    base = (uint32_t)__shfl((int)base, 63);
+    return base + x - n;
+}
+
 // Per-tag reductions (a count or a maximum per member of a batch) over a
 // range of items.  One global atomic per item on a per-member word would
 // serialize on that word (a member's items all hit it), so each workgroup
