@@ -120,32 +120,33 @@ __device__ __forceinline__ uint32_t sel4(const uint32_t *l, int i) {
 }
 
 // phA / phB: the output positions that hold a's / b's leaves (the truth-table
-// phases of rewrite.cu:195-201, read off the walk itself)
+// phases of rewrite.cu:195-201, read off the walk itself).  Selects only, no
+// branches: c is written whether or not the union fits in 4 leaves (the
+// return value says), so a failed merge leaves leaves of a and b in c.
 __device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut &c, int &phA, int &phB) {
     int i = 0, k = 0, n = 0;
     phA = 0;
     phB = 0;
-    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    uint32_t o[4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const bool ha = i < a.nLeaves, hb = k < b.nLeaves;
         const uint32_t ai = sel4(a.leaves, i < 3 ? i : 3), bk = sel4(b.leaves, k < 3 ? k : 3);
-        const bool takeA = ha && (!hb || ai <= bk);
-        const bool takeB = hb && (!ha || bk <= ai);
-        if (ha || hb) o[s] = takeA ? ai : bk;
-        phA |= takeA ? 1 << s : 0;
-        phB |= takeB ? 1 << s : 0;
-        n += (ha || hb);
-        i += takeA;
-        k += takeB;
+        const bool takeA = ha & (!hb | (ai <= bk));
+        const bool takeB = hb & (!ha | (bk <= ai));
+        o[s] = (ha | hb) ? (takeA ? ai : bk) : 0u;
+        phA |= (int)takeA << s;
+        phB |= (int)takeB << s;
+        n += (int)(ha | hb);
+        i += (int)takeA;
+        k += (int)takeB;
     }
-    if (i < a.nLeaves || k < b.nLeaves) return 0;
     // positions past the count repeat the first leaf (the containment tests of
     // the slot replay rely on it; stored records zero them)
 #pragma unroll
     for (int p = 0; p < 4; p++) c.leaves[p] = p < n ? o[p] : o[0];
     c.nLeaves = n;
-    return 1;
+    return (i >= a.nLeaves) & (k >= b.nLeaves);
 }
 
 // swaps truth-table variables v and v + 1 (v in 0..2), without branches
@@ -380,8 +381,8 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
     mm = m;
     {
         // every lane runs the whole computation (no divergent branches; a
-        // failed merge leaves m as initialised, so every leaf read below is a
-        // valid id) and the status says which results the replay may use
+        // failed merge leaves leaves of a and b in m, so every leaf read below
+        // is a valid id) and the status says which results the replay may use
         const bool cand = a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4;
         int phA, phB;
         const int ok = merge_ordered(a, b, m, phA, phB);
