@@ -70,6 +70,14 @@ class Tuning(ctypes.Structure):
 
 
 GLS_TUNE_RW, GLS_TUNE_RWZ, GLS_TUNE_STRASH, GLS_TUNE_BALANCE = 0, 1, 2, 3
+GLS_NET_CONST1, GLS_NET_CI, GLS_NET_AND, GLS_NET_CO = 0, 1, 2, 3
+
+
+class Net(ctypes.Structure):
+    _fields_ = [("nObjs", ctypes.c_int32), ("nCis", ctypes.c_int32), ("nCos", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("kind", ctypes.c_void_p), ("fanin0", ctypes.c_void_p),
+                ("fanin1", ctypes.c_void_p), ("compl0", ctypes.c_void_p), ("compl1", ctypes.c_void_p),
+                ("ci", ctypes.c_void_p), ("co", ctypes.c_void_p)]
 
 
 class CecInfo(ctypes.Structure):
@@ -114,6 +122,7 @@ _SIGS = {
     "gls_batch_strash": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_rewrite": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "gls_batch_balance": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "gls_aig_from_net": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.POINTER(_P)]),
     "gls_ctx_get_tuning": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_void_p]),
     "gls_ctx_set_tuning": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_void_p]),
     "gls_autotune": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -238,6 +247,23 @@ class Context:
         self.check(lib().gls_batch_balance(self._h, self._handles(aigs), len(aigs), 1, 1 if update_level else 0),
                    "gls_batch_balance")
         return aigs
+
+    def from_net(self, kind, fanin0, fanin1, compl0, compl1, ci, co):
+        """gls_aig_from_net: an object network (GLS_NET_* kinds, fanin object
+        indices and complement bits, CI and CO object lists) numbered as the
+        ABC patch's gget numbers it."""
+        kind = np.ascontiguousarray(kind, dtype=np.uint8)
+        f0 = np.ascontiguousarray(fanin0, dtype=np.int32)
+        f1 = np.ascontiguousarray(fanin1, dtype=np.int32)
+        c0 = np.ascontiguousarray(compl0, dtype=np.uint8)
+        c1 = np.ascontiguousarray(compl1, dtype=np.uint8)
+        ci = np.ascontiguousarray(ci, dtype=np.int32)
+        co = np.ascontiguousarray(co, dtype=np.int32)
+        net = Net(len(kind), len(ci), len(co), 0, _ptr(kind), _ptr(f0), _ptr(f1), _ptr(c0), _ptr(c1),
+                  _ptr(ci) if len(ci) else None, _ptr(co) if len(co) else None)
+        h = _P()
+        self.check(lib().gls_aig_from_net(self._h, ctypes.byref(net), ctypes.byref(h)), "gls_aig_from_net")
+        return Aig(self, h)
 
     def get_tuning(self, objects):
         """The pass settings used for graphs (or batch arenas) of `objects` objects."""

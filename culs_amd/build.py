@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-I" + SRC, "-I" + os.path.join(HERE, "..", "include")]
 
 LIB_SOURCES = ["strash.hip", "rewrite.hip", "batch.hip", "balance.hip", "sim.hip", "capi.cpp", "host_io.cpp",
-               "cpu_strash.cpp", "server.cpp", "tune.cpp"]
+               "cpu_strash.cpp", "server.cpp", "tune.cpp", "netbridge.cpp"]
 
 
 def _run(cmd):

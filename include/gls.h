@@ -115,6 +115,34 @@ int gls_aig_read_aiger_mem(gls_ctx *ctx, const char *buf, int64_t len, gls_aig *
 /* Replaces AIGMan::saveFile (src/aig_manager.cu:231-267). */
 int gls_aig_write_aiger(gls_aig *aig, const char *path);
 
+/* ---- in-process networks (the ABC patch's gget) ------------------------ */
+/* An AIG network as a caller holds it in memory (ABC's strashed Abc_Ntk_t in
+ * the reference's ABC patch): objects 0 .. nObjs-1, each a constant-1 node, a
+ * combinational input, an AND or a combinational output; ANDs and outputs
+ * name their fanin objects with complement bits; ci / co list the inputs and
+ * outputs in the network's own order.  Replaces AbcNtkToGpuMan
+ * (src/abc_patch/abc_patch_transform.cpp:5-87): PI k is ci[k]; the ANDs are
+ * numbered in depth-first post-order from the outputs (output order, fanin 0
+ * before fanin 1, each node once; nodes no output reaches are left out), the
+ * order Abc_NtkDfs gives them; the constant-1 node is the constant.  A fanin
+ * that is not a constant, a listed input or an AND, or a cycle, gives
+ * GLS_ERR_ARG; an object of kind GLS_NET_CI that is not in ci and that
+ * nothing references is allowed (unused object ids).  The other direction (GpuManToAbcNtk,
+ * abc_patch_transform.cpp:89-149) is gls_aig_to_arrays followed by one
+ * Abc_AigAnd per AND in id order (INTEGRATION.md shows both). */
+#define GLS_NET_CONST1 0
+#define GLS_NET_CI 1
+#define GLS_NET_AND 2
+#define GLS_NET_CO 3
+typedef struct {
+    int32_t nObjs, nCis, nCos, pad;
+    const uint8_t *kind;               /* nObjs entries, GLS_NET_* */
+    const int32_t *fanin0, *fanin1;    /* nObjs entries: fanin objects of an AND; a CO uses fanin0 */
+    const uint8_t *compl0, *compl1;    /* nObjs entries: complement bits of those edges */
+    const int32_t *ci, *co;            /* the CI objects and the CO objects, in order */
+} gls_net;
+int gls_aig_from_net(gls_ctx *ctx, const gls_net *net, gls_aig **out);
+
 /* ---- the hot path ----------------------------------------------------- */
 /* Structural hashing + dangling-node removal on the device.  Replaces
  * Aig::strash (src/aig/strash.cuh:12-14, strash.cu:190-314) behind
