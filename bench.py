@@ -55,16 +55,17 @@ def cut_kernel_name(mode):
     mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     return {culs_amd.GLS_RW_XCD_CUTS: "k_cuts_xcd", culs_amd.GLS_RW_LEVEL_CUTS: "k_cuts_level",
             culs_amd.GLS_RW_COOP_CUTS: "k_cuts_levels", culs_amd.GLS_RW_DATAFLOW_CUTS: "k_cuts_dataflow",
-            culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair"}.get(mode, "k_cuts_triple")
+            culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair", culs_amd.GLS_RW_PLAN_CUTS: "k_cuts_plan"}.get(mode, "k_cuts_triple")
 
 
-def cut_launches(mode, nlev):
-    """Cut-enumeration launches per rw pass over nlev AND levels."""
+def cut_launches(mode, nlev, levels=3):
+    """Cut-enumeration launches per rw pass over nlev AND levels (`levels` per
+    launch of the planned schedule)."""
     import culs_amd
     mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     if mode in (culs_amd.GLS_RW_XCD_CUTS, culs_amd.GLS_RW_COOP_CUTS, culs_amd.GLS_RW_DATAFLOW_CUTS):
         return 1
-    per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2}.get(mode, 3)
+    per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2, culs_amd.GLS_RW_PLAN_CUTS: levels}.get(mode, 3)
     return (nlev + per - 1) // per
 
 
@@ -96,6 +97,8 @@ def main():
                     help="cut-enumeration schedule (gls_rewrite flags): 0 three levels per launch, 2 "
                          "cooperative, 4 dataflow, 8 one XCD, 16 per level, 32 two levels per launch; +128 "
                          "launches the schedule from one captured HIP graph")
+    ap.add_argument("--cut-levels", type=int, default=0,
+                    help="levels per launch of the planned schedule (gls_tuning.cut_levels; 0: the default)")
     ap.add_argument("--stage", choices=["rw", "cuts", "strash"], default="rw",
                     help="cuts: time only reorder + cut enumeration (gls_enumerate_cuts); strash: only the "
                          "strash of the generated input; both for counter runs")
@@ -140,6 +143,9 @@ def main():
     ctx = culs_amd.Context(0)
     if not ctx.has_device:
         raise SystemExit("no GPU")
+    if a.cut_levels:
+        ctx.set_tuning(-1, cut_levels=a.cut_levels)
+    cut_levels = ctx.get_tuning(1)["cut_levels"]
     t0 = time.time()
     raw = ctx.gen_geometric(a.pis, a.nands, a.pos, a.window, a.p, a.seed)
     g_in = raw.arrays() if not a.no_check else None
@@ -166,7 +172,7 @@ def main():
         c = src.copy()
         if a.stage == "cuts":
             c.reorder()
-            c.enumerate_cuts()
+            c.enumerate_cuts(a.cut_mode)
             return c
         c.rewrite(cut_mode=a.cut_mode)
         return c
@@ -225,7 +231,7 @@ def main():
                            "profiles/k_cuts_triple_geom200k_counters.md)",
                 "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_workload else None,
                 "traffic_note": TRAFFIC_NOTE if default_workload else None,
-                "kernel": cut_kernel_name(a.cut_mode), "launches_per_step": cut_launches(a.cut_mode, nlev),
+                "kernel": cut_kernel_name(a.cut_mode), "launches_per_step": cut_launches(a.cut_mode, nlev, cut_levels),
                 "timed_launches": ks,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3), "bytes_per_launch": int(bytes_per_launch),
                 "bytes_per_node": B_CUT_PER_AND,

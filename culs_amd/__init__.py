@@ -23,6 +23,7 @@ GLS_RW_LEVEL_CUTS = 16
 GLS_RW_PAIR_CUTS = 32
 GLS_RW_FULL_STRASH = 64
 GLS_RW_GRAPH_CUTS = 128
+GLS_RW_PLAN_CUTS = 256
 
 
 class GlsError(RuntimeError):
@@ -63,7 +64,7 @@ class OpInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("cut_wide", ctypes.c_int32), ("rw_sweep_width", ctypes.c_int32),
                 ("strash_table_factor", ctypes.c_int32), ("strash_wave_batch", ctypes.c_int32),
-                ("balance_narrow_max", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("balance_narrow_max", ctypes.c_int32), ("cut_levels", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -370,10 +371,10 @@ class Aig:
         self.ctx.check(lib().gls_reorder(self._h, _ptr(lev)), "gls_reorder")
         return lev[:self.stats().nObjs]
 
-    def enumerate_cuts(self):
+    def enumerate_cuts(self, cut_mode=0):
         s = self.stats()
         cuts = np.zeros(s.nObjs * 40, np.uint32)
-        self.ctx.check(lib().gls_enumerate_cuts(self._h, _ptr(cuts)), "gls_enumerate_cuts")
+        self.ctx.check(lib().gls_enumerate_cuts_sched(self._h, int(cut_mode), _ptr(cuts)), "gls_enumerate_cuts_sched")
         return cuts
 
     def evaluate(self, use_zeros=False):

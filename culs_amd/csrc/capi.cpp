@@ -542,8 +542,13 @@ static int level_starts_host(gls_aig *a, std::vector<int32_t> &starts) {
     return GLS_OK;
 }
 
-extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) {
+extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) { return gls_enumerate_cuts_sched(a, 0, cuts); }
+
+extern "C" int gls_enumerate_cuts_sched(gls_aig *a, int32_t flags, uint32_t *cuts) {
     if (!a || !cuts) return GLS_ERR_ARG;
+    if (flags & ~(GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS | GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS |
+                  GLS_RW_GRAPH_CUTS | GLS_RW_PLAN_CUTS))
+        return gls_fail(a->ctx, GLS_ERR_ARG, "gls_enumerate_cuts_sched: unknown flags 0x%x", flags);
     gls_ctx *ctx = a->ctx;
     GLS_TRY(aig_ensure_device(a));
     if (!a->dev_level_valid) return gls_fail(ctx, GLS_ERR_ARG, "levels unknown: call gls_reorder first");
@@ -553,7 +558,11 @@ extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) {
     uint32_t *d_cuts = pool_get<uint32_t>(ctx, n * 40);
     int32_t *d_nref = pool_get<int32_t>(ctx, n);
     if (!d_cuts || !d_nref) { pool_put(ctx, d_cuts); pool_put(ctx, d_nref); return gls_fail(ctx, GLS_ERR_NOMEM, "cuts"); }
-    int rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref, 0);
+    int rc = GLS_OK;
+    {
+        TuneScope ts(ctx, a->nObjs);
+        rc = dev_enumerate_cuts(ctx, a->dev, starts, d_cuts, d_nref, flags);
+    }
     if (rc == GLS_OK) rc = stream_copy(ctx, cuts, d_cuts, n * 40 * sizeof(uint32_t), hipMemcpyDeviceToHost);
     pool_put(ctx, d_cuts);
     pool_put(ctx, d_nref);

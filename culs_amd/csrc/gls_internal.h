@@ -71,7 +71,7 @@ struct gls_ctx {
 };
 
 // the defaults of gls_tuning
-constexpr gls_tuning kTuningDefault{2048, 1024, 4, 4, 4096, 0};
+constexpr gls_tuning kTuningDefault{2048, 1024, 4, 4, 4096, 3};
 int tuning_bucket(int64_t objects);
 // the settings of the running pass (the active set, or the bucket of `objects`)
 inline const gls_tuning &tuning_for(const gls_ctx *ctx, int64_t objects) {

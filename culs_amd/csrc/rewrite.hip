@@ -345,14 +345,15 @@ enum : int { REC_PLAIN = 0, REC_TAGGED = 1 };
 // Cut set of node idx (fanin literals l0, l1), computed by one whole wavefront
 // (lane = thread lane).  MODE REC_PLAIN: the fanins' 8-slot records (20 bytes
 // per slot) start at ra and rb, in global memory or LDS, and the result goes to
-// out in the same format.  MODE REC_TAGGED: fanin cut sets are polled in the
-// fence-free format and the result is written in it (tcuts, 24 bytes per slot).
+// out in the same format (and to out2 as well when it is not null).  MODE
+// REC_TAGGED: fanin cut sets are polled in the fence-free format and the result
+// is written in it (tcuts, 24 bytes per slot).
 // Returns false if a poll exceeded its bound (*err set), true otherwise.
 template <int MODE>
 __device__ __forceinline__ bool cut_node(const int idx, const int lane, const uint32_t l0, const uint32_t l1,
                                          const uint32_t *ra, const uint32_t *rb, uint32_t *out,
                                          const int32_t *__restrict__ nRef, unsigned long long *tcuts, uint32_t *err,
-                                         const uint8_t *__restrict__ sigBits) {
+                                         const uint8_t *__restrict__ sigBits, uint32_t *out2 = nullptr) {
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
     DCut a, b;
     if (MODE == REC_TAGGED) {
@@ -466,6 +467,10 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         } else {
             uint32_t *o = out + lane * 5;
             o[0] = o0; o[1] = o1; o[2] = o2; o[3] = o3; o[4] = meta;
+            if (out2) {
+                uint32_t *q = out2 + lane * 5;
+                q[0] = o0; q[1] = o1; q[2] = o2; q[3] = o3; q[4] = meta;
+            }
         }
     }
     return true;
@@ -618,6 +623,78 @@ __global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_triple(int32_t s0_, int32_t
         cut_node<REC_PLAIN>((int)v, lane, l0, l1, rec_or_lds(a, cuts, s_rec, b0, b1, b2, m0, m1, s0, s1),
                             rec_or_lds(b, cuts, s_rec, b0, b1, b2, m0, m1, s0, s1), cuts + (size_t)v * NCUT * 5, nRef,
                             nullptr, nullptr, sigBits);
+}
+
+// ---- planned schedule (GLS_RW_PLAN_CUTS): K levels per launch
+// A launch covers K consecutive levels (positions 0..K-1).  The in-launch
+// cone of a node (its transitive fanins inside the launch) is fixed by the
+// graph alone, so the cones are listed once per pass before the launches:
+//   - every node of the top position is a root; going down the positions, a
+//     node that no higher root's cone contains becomes a root as well;
+//   - a root's workgroup recomputes its cone position by position (one
+//     wavefront per node, records in LDS, a barrier between positions), and
+//     every cone node is stored to the global records by exactly one of the
+//     workgroups that recompute it (the smallest root id), the root by its own;
+//   - position-0 nodes that no cone contains are packed four to a workgroup.
+// Each workgroup reads a header and its entries (node, fanin literals, where
+// each fanin's record is: an LDS slot of this task or the global records)
+// with scalar loads, so no fanin is looked up at run time.  Every record is
+// computed by the deterministic cut_node, so the results are bit-identical to
+// one launch per level.  On the 10M-AND workload with K = 3 a launch runs
+// ~328 workgroups instead of k_cuts_triple's ~617.
+// Header (uint4): x = first entry, y = entry count | top position << 16,
+//   z, w = node count of positions 0..3 and 4..7 (8 bits each).
+// Entry (uint4): x = node id | stored-globally << 31 | kept-in-LDS << 30,
+//   y, z = fanin literals, w = fanin sources (8 bits each: an entry index of
+//   this task, or PLAN_GLOBAL).
+#include "cut_plan.h"
+
+// One planned task per workgroup (the launch's tasks start at `base`).
+__global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_plan(const uint4 *__restrict__ heads,
+                                                          const uint4 *__restrict__ ents, int32_t base, int32_t K,
+                                                          const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                          const uint8_t *__restrict__ sigBits) {
+    __shared__ uint32_t s_rec[PLAN_MAX_CONE][NCUT * 5];
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint4 h = heads[base + blockIdx.x];
+    const uint32_t top = h.y >> 16;
+    // issue priority by chain length: the roots of the top position bound the launch
+    if (top + 1 >= (uint32_t)K && K > 1) __builtin_amdgcn_s_setprio(2);
+    else if (top > 0) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+    uint32_t start = 0;
+    bool first = true;
+    for (int p = 0; p < K; p++) {
+        const uint32_t cnt = (p < 4 ? h.z >> (8 * p) : h.w >> (8 * (p - 4))) & 0xFFu;
+        if (cnt == 0) continue;                 // uniform per workgroup
+        if (!first) {
+            // this wavefront's LDS records are written before any wavefront
+            // passes the barrier and reads them
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
+        first = false;
+        for (uint32_t k = w; k < cnt; k += CUT3_BLOCK / 64) {
+            const uint4 e = ents[h.x + start + k];
+            const uint32_t id = e.x & 0x3FFFFFFFu;
+            const uint32_t s0 = e.w & 0xFFu, s1 = (e.w >> 8) & 0xFFu;
+            const uint32_t *ra = s0 == PLAN_GLOBAL ? cuts + (size_t)LID(e.y) * NCUT * 5 : s_rec[s0];
+            const uint32_t *rb = s1 == PLAN_GLOBAL ? cuts + (size_t)LID(e.z) * NCUT * 5 : s_rec[s1];
+            uint32_t *glob = cuts + (size_t)id * NCUT * 5;
+            const bool stored = (e.x >> 31) != 0u, kept = ((e.x >> 30) & 1u) != 0u;
+            // separate call sites, so that the LDS record is written through an
+            // LDS pointer (ds_write, which the barrier waits for; a pointer that
+            // is LDS or global at run time would make it a flat store that the
+            // barrier does not wait for)
+            if (kept)
+                cut_node<REC_PLAIN>((int)id, lane, e.y, e.z, ra, rb, s_rec[start + k], nRef, nullptr, nullptr, sigBits,
+                                    stored ? glob : nullptr);
+            else
+                cut_node<REC_PLAIN>((int)id, lane, e.y, e.z, ra, rb, glob, nRef, nullptr, nullptr, sigBits);
+        }
+        start += cnt;
+    }
 }
 
 // All levels in one cooperative launch: the wavefronts of the grid stride over
@@ -1505,6 +1582,65 @@ static const bool kSampleLaunches = [] {
     return !(e && e[0] == '0');
 }();
 
+// Task lists of the planned schedule for nL launches of K levels each (lv:
+// per launch the first ids of its K levels, then its end; K + 1 entries):
+// heads / ents on the device, toff[j] = first task of launch j on the host
+// (nL + 1 entries).  lv must stay alive until this returns (its upload is
+// complete by then).
+static int build_cut_plan(gls_ctx *ctx, Tmp &t, const DevGraph &g, int32_t K, const std::vector<int32_t> &lv,
+                          uint4 *&heads, uint4 *&ents, std::vector<uint32_t> &toff) {
+    hipStream_t st = ctx->stream;
+    const int32_t n = g.nObjs, nL = (int32_t)(lv.size() / (size_t)(K + 1));
+    int32_t *dlv = t.get<int32_t>(lv.size());
+    int32_t *nodeLaunch = t.get<int32_t>((size_t)n);
+    uint32_t *own = t.get<uint32_t>((size_t)n);
+    uint8_t *coneSize = t.get<uint8_t>((size_t)n);
+    unsigned long long *flag = t.get<unsigned long long>((size_t)n + 1);
+    unsigned long long *S = t.get<unsigned long long>((size_t)n + 1);
+    uint32_t *cnt = t.get<uint32_t>((size_t)nL + 1), *dOff = t.get<uint32_t>((size_t)nL + 1);
+    if (!t.ok) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: plan storage");
+    GLS_HIP(ctx, hipMemcpyAsync(dlv, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice, st));
+    GLS_HIP(ctx, hipMemsetAsync(nodeLaunch, 0xFF, sizeof(int32_t) * (size_t)n, st));
+    GLS_HIP(ctx, hipMemsetAsync(own, 0xFF, sizeof(uint32_t) * (size_t)n, st));
+    k_plan_fill<<<nL, kBlock, 0, st>>>(K, dlv, nodeLaunch);
+    for (int p = K - 1; p >= 1; p--)
+        k_plan_roots<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, K, p, g.f0, g.f1, dlv, nodeLaunch, own, coneSize);
+    k_plan_flags<<<grid_for((int64_t)n + 1), kBlock, 0, st>>>(n, K, dlv, nodeLaunch, own, flag);
+    GLS_HIP(ctx, hipGetLastError());
+    size_t tb = 0, tb2 = 0;
+    GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb, flag, S, 0ull, (size_t)n + 1,
+                                         rocprim::plus<unsigned long long>(), st));
+    GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb2, cnt, dOff, 0u, (size_t)nL + 1, rocprim::plus<uint32_t>(), st));
+    void *tmp = t.get<uint8_t>(std::max<size_t>(std::max(tb, tb2), 1));
+    if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: plan scan storage");
+    GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, flag, S, 0ull, (size_t)n + 1, rocprim::plus<unsigned long long>(),
+                                         st));
+    k_plan_counts<<<blocks(nL + 1, kBlock), kBlock, 0, st>>>(nL, K, dlv, S, cnt);
+    GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb2, cnt, dOff, 0u, (size_t)nL + 1, rocprim::plus<uint32_t>(), st));
+    toff.assign((size_t)nL + 1, 0u);
+    GLS_TRY(stream_copy(ctx, toff.data(), dOff, sizeof(uint32_t) * ((size_t)nL + 1), hipMemcpyDeviceToHost));
+    const size_t nT = toff[nL];
+    uint32_t *ecnt = t.get<uint32_t>(nT + 1), *eoff = t.get<uint32_t>(nT + 1);
+    heads = t.get<uint4>(std::max<size_t>(nT, 1));
+    size_t tb3 = 0;
+    GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tb3, ecnt, eoff, 0u, nT + 1, rocprim::plus<uint32_t>(), st));
+    void *tmp3 = t.get<uint8_t>(std::max<size_t>(tb3, 1));
+    if (!ecnt || !eoff || !heads || !tmp3) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: plan tasks (%zu)", nT);
+    GLS_HIP(ctx, hipMemsetAsync(ecnt, 0, sizeof(uint32_t) * (nT + 1), st));
+    k_plan_ecount<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, K, dlv, nodeLaunch, own, S, dOff, coneSize, ecnt);
+    GLS_HIP(ctx, rocprim::exclusive_scan(tmp3, tb3, ecnt, eoff, 0u, nT + 1, rocprim::plus<uint32_t>(), st));
+    // entries: every node once as a root or packed node, plus the cone
+    // members recomputed by other roots; bounded by the cone sizes' sum
+    uint32_t nE = 0;
+    GLS_TRY(stream_copy(ctx, &nE, eoff + nT, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    ents = t.get<uint4>(std::max<size_t>(nE, 1));
+    if (!ents) return gls_fail(ctx, GLS_ERR_NOMEM, "cuts: plan entries (%u)", nE);
+    k_plan_scatter<<<grid_for(n), kBlock, 0, st>>>(n, g.nPIs, K, g.f0, g.f1, dlv, nodeLaunch, own, S, dOff, eoff,
+                                                   heads, ents);
+    GLS_HIP(ctx, hipGetLastError());
+    return GLS_OK;
+}
+
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart, uint32_t *d_cuts,
                        int32_t *d_nref, int mode, const uint8_t *sigBits, int *ranMode) {
     int ranDummy = 0;
@@ -1517,6 +1653,13 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
     ctx->ksamp_launches.clear();
     ctx->cut_graph_ms = 0;
     ctx->cut_graph_nodes = 0;
+    // The planned schedule is switched off: its kernel made an illegal memory
+    // access on the GPU in the schedule tests, also after its LDS records were
+    // written through LDS pointers with a wait before each barrier, and the
+    // cause is not found (the task lists pass every bounds check in a host
+    // build of the plan kernels).  It is refused before anything is launched.
+    if (mode & GLS_RW_PLAN_CUTS)
+        return gls_fail(ctx, GLS_ERR_ARG, "GLS_RW_PLAN_CUTS is disabled (its kernel faulted the GPU; cause not found)");
     const bool graphed = (mode & GLS_RW_GRAPH_CUTS) != 0;
     mode &= ~GLS_RW_GRAPH_CUTS;
     GLS_HIP(ctx, hipMemsetAsync(d_nref, 0, sizeof(int32_t) * n, st));
@@ -1637,10 +1780,13 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // with a level wider than cut_wide nodes fills the GPU by itself: the
         // recomputed cones would only add work there, so its levels get one
         // launch each.
-        const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : 3;
+        // entries hold node ids in 30 bits
+        const bool planned = mode == GLS_RW_PLAN_CUTS && n < (1 << 30);
+        const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2
+                         : planned ? std::min(std::max(tuning_for(ctx, n).cut_levels, 1), PLAN_MAX_LEVELS) : 3;
         const int32_t cutWide = tuning_for(ctx, n).cut_wide;
-        ran = step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS : 0;
-        struct CutLaunch { int kind; int32_t s, m, m2, e; };
+        ran = planned ? GLS_RW_PLAN_CUTS : step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS : 0;
+        struct CutLaunch { int kind; int32_t s, m, m2, e, l; };
         std::vector<CutLaunch> plan;
         plan.reserve((size_t)L / step + 1);
         for (int l = 1; l < L; l += step) {
@@ -1648,12 +1794,14 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
             const int32_t m2 = levelStart[std::min(l + 2, L)];
             const int32_t e = levelStart[std::min(l + step, L)];
             if (e <= s) continue;
-            if (step > 1 && std::max(std::max(m - s, m2 - m), e - m2) > cutWide) {
+            int32_t widest = 0;
+            for (int j = l; j < std::min(l + step, L); j++) widest = std::max(widest, levelStart[j + 1] - levelStart[j]);
+            if (step > 1 && widest > cutWide) {
                 for (int j = l; j < std::min(l + step, L); j++)
                     if (levelStart[j + 1] > levelStart[j])
-                        plan.push_back({1, levelStart[j], levelStart[j + 1], levelStart[j + 1], levelStart[j + 1]});
+                        plan.push_back({1, levelStart[j], levelStart[j + 1], levelStart[j + 1], levelStart[j + 1], j});
             } else {
-                plan.push_back({step, s, m, m2, e});
+                plan.push_back({step, s, m, m2, e, l});
             }
         }
         // sampled timing of the schedule's own kernel: a window of up to
@@ -1665,6 +1813,19 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // GLS_RW_GRAPH_CUTS: the same launches captured into a HIP graph (kernel
         // nodes only), instantiated and launched once; the host time of the
         // capture and the instantiation is reported apart
+        // planned schedule: the task lists of the three-level launches, built
+        // before any launch (and before a capture begins)
+        Tmp pt{ctx};
+        std::vector<int32_t> lv;
+        std::vector<uint32_t> toff;
+        uint4 *heads = nullptr, *ents = nullptr;
+        if (planned && step > 1) {
+            for (const CutLaunch &c : plan)
+                if (c.kind == step)
+                    for (int p = 0; p <= step; p++) lv.push_back(levelStart[std::min(c.l + p, L)]);
+            if (!lv.empty()) GLS_TRY(build_cut_plan(ctx, pt, g, step, lv, heads, ents, toff));
+        }
+        size_t j3 = 0;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         const auto tg0 = std::chrono::steady_clock::now();
@@ -1672,7 +1833,10 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         int win = 0;
         for (size_t i = 0; i < plan.size(); i++) {
             const CutLaunch &c = plan[i];
-            const dim3 grid(c.kind > 1 ? (unsigned)(c.e - c.s) : blocks(c.e - c.s, CUTL_BLOCK / 64));
+            const bool pl = planned && step > 1 && c.kind == step;
+            const uint32_t tbase = pl ? toff[j3] : 0u, tcount = pl ? toff[j3 + 1] - toff[j3] : 0u;
+            if (pl) ++j3;
+            const dim3 grid(pl ? tcount : c.kind > 1 ? (unsigned)(c.e - c.s) : blocks(c.e - c.s, CUTL_BLOCK / 64));
             hipEvent_t evStart = nullptr, evStop = nullptr;
             if (kSampleLaunches && !graphed && win == 0 && c.kind == step && i % KSAMPLE_EVERY == 0) {
                 const size_t k = 2 * (size_t)ctx->ksamp_n;
@@ -1695,7 +1859,11 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                     win = 0;
                     evStop = ctx->kev[2 * (size_t)ctx->ksamp_n - 1];
                 }
-                if (c.kind == 3)
+                if (pl)
+                    hipExtLaunchKernelGGL(k_cuts_plan, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0,
+                                          (const uint4 *)heads, (const uint4 *)ents, (int32_t)tbase, (int32_t)step,
+                                          (const int32_t *)d_nref, d_cuts, sigBits);
+                else if (c.kind == 3)
                     hipExtLaunchKernelGGL(k_cuts_triple, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0, c.s, c.m,
                                           c.m2, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
                                           (const int32_t *)d_nref, d_cuts, sigBits);
@@ -1707,6 +1875,8 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                     hipExtLaunchKernelGGL(k_cuts_level, grid, dim3(CUTL_BLOCK), 0, st, evStart, evStop, 0, c.s,
                                           c.e - c.s, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
                                           (const int32_t *)d_nref, d_cuts, sigBits);
+            } else if (pl) {
+                k_cuts_plan<<<grid, CUT3_BLOCK, 0, st>>>(heads, ents, (int32_t)tbase, step, d_nref, d_cuts, sigBits);
             } else if (c.kind == 3) {
                 k_cuts_triple<<<grid, CUT3_BLOCK, 0, st>>>(c.s, c.m, c.m2, g.f0, g.f1, d_nref, d_cuts, sigBits);
             } else if (c.kind == 2) {
@@ -1822,7 +1992,8 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         int cutRan = 0;
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
                                    flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS |
-                                            GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS | GLS_RW_GRAPH_CUTS),
+                                            GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS | GLS_RW_GRAPH_CUTS |
+                                            GLS_RW_PLAN_CUTS),
                                    sigBits, &cutRan));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr};

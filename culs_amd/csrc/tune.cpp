@@ -18,7 +18,7 @@ namespace {
 bool tuning_valid(const gls_tuning &t) {
     return t.cut_wide >= 1 && t.rw_sweep_width >= 1 && (t.strash_table_factor == 4 || t.strash_table_factor == 8) &&
            t.strash_wave_batch >= 1 && t.strash_wave_batch <= 64 && t.balance_narrow_max >= 1 &&
-           t.balance_narrow_max <= (1 << 24);
+           t.balance_narrow_max <= (1 << 24) && t.cut_levels >= 1 && t.cut_levels <= 6;
 }
 
 // one timed run of the command over fresh device copies of the inputs
@@ -58,7 +58,6 @@ extern "C" int gls_ctx_set_tuning(gls_ctx *ctx, int64_t objects, const gls_tunin
     if (!ctx || !t) return GLS_ERR_ARG;
     if (!tuning_valid(*t)) return gls_fail(ctx, GLS_ERR_ARG, "gls_ctx_set_tuning: a setting is out of range");
     gls_tuning v = *t;
-    v.pad = 0;
     if (objects < 0) {
         for (auto &x : ctx->tune) x = v;
     } else {

@@ -178,6 +178,11 @@ int gls_strash(gls_aig *aig, int32_t flags);
  * pass's cut launches into a HIP graph and launch that once (for measuring
  * graph launch against stream launches; the results are identical). */
 #define GLS_RW_GRAPH_CUTS 128
+/* GLS_RW_PLAN_CUTS: cut_levels levels per launch from cone lists built once
+ * per pass on the device (a cone's workgroup also stores the nodes it
+ * recomputes, one owner each).  Currently DISABLED: refused with GLS_ERR_ARG,
+ * because its kernel faulted the GPU in testing and the cause is not found. */
+#define GLS_RW_PLAN_CUTS 256
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing
@@ -213,8 +218,8 @@ int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_
  * room its tables have; every setting gives the same results.  A context
  * keeps one set per size bucket (floor(log2(objects)) of the graph or batch
  * arena a pass runs on), initially the defaults below.
- *   cut_wide: a group of three levels whose widest level has more nodes runs
- *     one cut launch per level (default 2048);
+ *   cut_wide: a group of levels (three, or cut_levels when planned) whose
+ *     widest level has more nodes runs one cut launch per level (default 2048);
  *   rw_sweep_width: rw finishes with the incremental strash while the input
  *     has at most this many objects per level on average, else with the
  *     wave sweep (default 1024);
@@ -224,9 +229,11 @@ int gls_batch_balance(gls_ctx *ctx, gls_aig **aigs, int32_t count, int32_t sort_
  *     (default 4);
  *   balance_narrow_max: balance levels of at most this many covers are
  *     rebuilt in one persistent workgroup, wider ones by grid-wide phase
- *     kernels (default 4096). */
+ *     kernels (default 4096);
+ *   cut_levels: levels per cut launch of the planned schedule
+ *     (GLS_RW_PLAN_CUTS, currently disabled), 1-6 (default 3). */
 typedef struct {
-    int32_t cut_wide, rw_sweep_width, strash_table_factor, strash_wave_batch, balance_narrow_max, pad;
+    int32_t cut_wide, rw_sweep_width, strash_table_factor, strash_wave_batch, balance_narrow_max, cut_levels;
 } gls_tuning;
 /* The settings a pass over `objects` objects uses. */
 int gls_ctx_get_tuning(gls_ctx *ctx, int64_t objects, gls_tuning *out);
@@ -291,6 +298,8 @@ int gls_reorder(gls_aig *aig, uint32_t *level_out);
  * cut stage, rewrite.cu:689-705).  cuts: nObjs*8 records of 5 uint32:
  * leaves[4], meta = truth | value<<16 | nLeaves<<27 | used<<30. */
 int gls_enumerate_cuts(gls_aig *aig, uint32_t *cuts);
+/* The same under a cut schedule of gls_rewrite's flags (GLS_RW_*_CUTS). */
+int gls_enumerate_cuts_sched(gls_aig *aig, int32_t flags, uint32_t *cuts);
 /* Per-node best replacement of a reordered AIG (EvaluateNode, rewrite.cu:378-447).
  * best: nObjs*4 int32 {cut slot or -1, subgraph index, gain, root level}. */
 int gls_evaluate(gls_aig *aig, int32_t use_zeros, int32_t *best);

@@ -8,6 +8,11 @@ and long runs of one-node levels at every offset within a group, so groups of
 every shape occur: the cut records must equal the oracle's, and rw must give
 the same AIG under every schedule, launched from the stream or from one
 captured HIP graph, and equal the oracle's rw.
+
+The planned schedule (GLS_RW_PLAN_CUTS) is switched off in the library: its
+kernel made an illegal memory access on the GPU in the parity test below (also
+after a fix), the cause is not found, and the library refuses the flag.  Its
+parity test stays, skipped, and a test checks the refusal.
 """
 import sys
 
@@ -71,10 +76,13 @@ def test_cut_schedule_parity(gpu_ctx, name, spec):
     b = a.copy()
     b.reorder()
     r = b.arrays()
-    cuts = b.enumerate_cuts()
     want_cuts = O.cuts(*r)
-    assert np.array_equal(cuts, want_cuts), "GPU cut sets differ (first node %d)" % (
-        int(np.argmax((cuts != want_cuts).reshape(-1, 40).any(axis=1))))
+
+    def same_cuts(cuts, what):
+        assert np.array_equal(cuts, want_cuts), "GPU cut sets differ under %s (first node %d)" % (
+            what, int(np.argmax((cuts != want_cuts).reshape(-1, 40).any(axis=1))))
+
+    same_cuts(b.enumerate_cuts(), "the default schedule")
     want, _ = O.rw(*s)
     outs = []
     G = culs_amd.GLS_RW_GRAPH_CUTS
@@ -88,3 +96,46 @@ def test_cut_schedule_parity(gpu_ctx, name, spec):
         outs.append(x.arrays())
     for o in outs:
         assert all(np.array_equal(p, q) for p, q in zip(o, want)), "GPU rw differs from the oracle"
+
+
+@pytest.mark.skip(reason="GLS_RW_PLAN_CUTS is disabled: its kernel faulted the GPU (illegal memory access) "
+                         "in this test, also after a fix; the cause is not found")
+@pytest.mark.parametrize("name,spec", CASES, ids=[c[0] for c in CASES])
+def test_planned_schedule_parity(gpu_ctx, name, spec):
+    """The planned schedule at every launch depth it offers, record by record."""
+    import culs_amd
+    log("planned schedule parity: %s" % name)
+    a = gpu_ctx.from_arrays(*layered(**spec)).strash()
+    b = a.copy()
+    b.reorder()
+    want_cuts = O.cuts(*b.arrays())
+    P = culs_amd.GLS_RW_PLAN_CUTS
+    levels = gpu_ctx.get_tuning(1)["cut_levels"]
+    try:
+        for k in (1, 2, 3, 4, 5, 6):
+            gpu_ctx.set_tuning(-1, cut_levels=k)
+            assert np.array_equal(b.enumerate_cuts(P), want_cuts), "%d levels per launch" % k
+            x = a.copy()
+            x.rewrite(cut_mode=P)
+            want, _ = O.rw(*a.arrays())
+            assert all(np.array_equal(p, q) for p, q in zip(x.arrays(), want))
+    finally:
+        gpu_ctx.set_tuning(-1, cut_levels=levels)
+
+
+def test_planned_schedule_refused(gpu_ctx):
+    """The disabled schedule is refused before anything is launched."""
+    import culs_amd
+    log("planned schedule refused")
+    a = gpu_ctx.from_arrays(*layered(seed=5, nPIs=16, widths=[8] * 6)).strash()
+    b = a.copy()
+    b.reorder()
+    with pytest.raises(culs_amd.GlsError, match="disabled"):
+        b.enumerate_cuts(culs_amd.GLS_RW_PLAN_CUTS)
+    with pytest.raises(culs_amd.GlsError, match="disabled"):
+        a.copy().rewrite(cut_mode=culs_amd.GLS_RW_PLAN_CUTS)
+    # the context still works afterwards
+    c = a.copy()
+    c.rewrite()
+    want, _ = O.rw(*a.arrays())
+    assert all(np.array_equal(p, q) for p, q in zip(c.arrays(), want))

@@ -6,7 +6,7 @@ import pytest
 import culs_amd
 
 DEFAULTS = {"cut_wide": 2048, "rw_sweep_width": 1024, "strash_table_factor": 4, "strash_wave_batch": 4,
-            "balance_narrow_max": 4096}
+            "balance_narrow_max": 4096, "cut_levels": 3}
 
 
 def test_defaults_and_buckets():
@@ -33,7 +33,7 @@ def test_every_bucket():
 
 @pytest.mark.parametrize("bad", [{"strash_table_factor": 3}, {"strash_table_factor": 16}, {"cut_wide": 0},
                                  {"strash_wave_batch": 0}, {"strash_wave_batch": 65}, {"balance_narrow_max": 0},
-                                 {"rw_sweep_width": -5}])
+                                 {"rw_sweep_width": -5}, {"cut_levels": 0}, {"cut_levels": 7}])
 def test_out_of_range_refused(bad):
     c = culs_amd.Context(-1)
     with pytest.raises(culs_amd.GlsError):
