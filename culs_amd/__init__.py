@@ -243,10 +243,11 @@ class Context:
                    "gls_batch_rewrite")
         return aigs
 
-    def batch_balance(self, aigs, update_level=True):
-        """`b` over every AIG of the list in one device pass (results as if alone)."""
-        self.check(lib().gls_batch_balance(self._h, self._handles(aigs), len(aigs), 1, 1 if update_level else 0),
-                   "gls_batch_balance")
+    def batch_balance(self, aigs, update_level=True, shuffle=False):
+        """`b` over every AIG of the list in one device pass (results as if
+        alone); shuffle (`b -s`) is refused by the batched pass."""
+        self.check(lib().gls_batch_balance(self._h, self._handles(aigs), len(aigs), 0 if shuffle else 1,
+                                           1 if update_level else 0), "gls_batch_balance")
         return aigs
 
     def from_net(self, kind, fanin0, fanin1, compl0, compl1, ci, co):
@@ -355,9 +356,10 @@ class Aig:
         self.ctx.check(lib().gls_rewrite(self._h, 1 if use_zeros else 0, flags), "gls_rewrite")
         return self
 
-    def balance(self, update_level=True):
-        """`b` (update_level: strash afterwards, as the command does; False is `gb`)."""
-        self.ctx.check(lib().gls_balance(self._h, 1, 1 if update_level else 0), "gls_balance")
+    def balance(self, update_level=True, shuffle=False):
+        """`b` (update_level: strash afterwards, as the command does; False is
+        `gb`); shuffle: `b -s` (equal levels in thrust's shuffle order)."""
+        self.ctx.check(lib().gls_balance(self._h, 0 if shuffle else 1, 1 if update_level else 0), "gls_balance")
         return self
 
     def last_op(self):

@@ -327,6 +327,48 @@ struct RecArgs {
     uint32_t *err;                    // 2: table full
     const uint32_t *tagIn;            // optional: per-node tag of the input (batched balance)
     Tables tagTab;                    // created id -> tag of the cover that made it
+    uint32_t shuffle;                 // 1: `b -s` input order (sortDecId = 0)
+};
+
+// thrust::shuffle(thrust::seq, ..., default_random_engine(seed)) as an index
+// map, the tie-break of `b -s` (balance.cu:301, 404-411).  The algorithm is
+// thrust's (rocThrust thrust/system/detail/generic/shuffle.inl, ROCm 7.2): a
+// 24-round Feistel bijection over the next power of two (at least 16) with
+// round keys from minstd_rand seeded by `seed` (0 taken as 1); position k of
+// the shuffled sequence holds the input whose index is the k-th value below
+// len that the bijection yields for 0, 1, 2, ...
+struct ThrustShuffle {
+    uint32_t key[24];
+    uint32_t lb, rb, lmask, rmask, span;
+    __device__ ThrustShuffle(uint32_t len, uint32_t seed) {
+        uint32_t x = seed % 2147483647u;
+        if (x == 0) x = 1;
+        uint32_t bits = 4;
+        if (len > 16) {
+            bits = 0;
+            for (uint32_t m = len - 1; m; m >>= 1) bits++;
+        }
+        lb = bits / 2;
+        rb = bits - lb;
+        lmask = (1u << lb) - 1u;
+        rmask = (1u << rb) - 1u;
+        span = 1u << bits;
+        for (int i = 0; i < 24; i++) {
+            x = (uint32_t)(((uint64_t)x * 48271u) % 2147483647u);
+            key[i] = x;
+        }
+    }
+    __device__ uint32_t operator()(uint32_t j) const {
+        uint32_t s0 = j >> rb, s1 = j & rmask;
+        for (int r = 0; r < 24; r++) {
+            const uint64_t prod = 0xD2B74407B1CE6E93ull * (uint64_t)s0;
+            const uint32_t hi = (uint32_t)(prod >> 32);
+            const uint32_t lo = ((uint32_t)prod << (rb - lb)) | (s1 >> lb);
+            s0 = ((hi ^ key[r]) ^ s1) & lmask;
+            s1 = lo & rmask;
+        }
+        return (s0 << rb) | s1;
+    }
 };
 
 __device__ __forceinline__ void rec_tag(const RecArgs &r, uint32_t c, uint32_t root) {
@@ -338,17 +380,32 @@ __device__ void rec_gather(const RecArgs &r, uint32_t c, uint32_t root) {
     uint32_t *A = r.arr + (size_t)c * r.maxLen, *LV = r.alv + (size_t)c * r.maxLen;
     const uint32_t len = r.coverLen[root], off = r.coverOff[root];
     r.alen[c] = len;
-    for (uint32_t i = 0; i < len; i++) {
-        const uint32_t lit = r.pool[off + i], id = lit >> 1;
+    auto put = [&](uint32_t k, uint32_t src) {
+        const uint32_t lit = r.pool[off + src], id = lit >> 1;
         const uint32_t nl = (int32_t)id <= r.nPIs ? lit : (r.newLit[id] ^ (lit & 1u));
-        A[i] = nl;
-        LV[i] = lit_level(r.t, nl, r.nPIs);
+        A[k] = nl;
+        LV[k] = lit_level(r.t, nl, r.nPIs);
+    };
+    if (r.shuffle) {
+        // `b -s`: the inputs in the order of the shuffle seeded with the
+        // cover's index in its batch
+        if (len > 0) {
+            const ThrustShuffle sh(len, c);
+            uint32_t k = 0;
+            for (uint32_t j = 0; j < sh.span && k < len; j++) {
+                const uint32_t src = sh(j);
+                if (src < len) put(k++, src);
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < len; i++) put(i, i);
     }
-    // level descending, then the reference's literal value descending
+    // level descending, then (`b`) the reference's literal value descending;
+    // insertion sort keeps equal keys in order, as thrust's sequential sort does
     for (uint32_t i = 1; i < len; i++) {
         const uint32_t x = A[i], xl = LV[i];
         int j = (int)i - 1;
-        while (j >= 0 && (LV[j] < xl || (LV[j] == xl && culs_val(A[j]) < culs_val(x)))) {
+        while (j >= 0 && (LV[j] < xl || (!r.shuffle && LV[j] == xl && culs_val(A[j]) < culs_val(x)))) {
             A[j + 1] = A[j];
             LV[j + 1] = LV[j];
             j--;
@@ -694,8 +751,12 @@ uint32_t bits_of(uint32_t x) {
 } // namespace
 
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn,
-                uint32_t **tagOut, std::vector<uint32_t> *splitTags) {
+                uint32_t **tagOut, std::vector<uint32_t> *splitTags, int sortDecId) {
     const bool tagged = tagIn && tagOut;
+    if (sortDecId != 0 && sortDecId != 1) return gls_fail(ctx, GLS_ERR_ARG, "balance: sortDecId must be 0 or 1");
+    // the shuffle is seeded with a cover's index in its batch, which in a
+    // batched arena is not its index when balanced alone
+    if (sortDecId == 0 && tagged) return gls_fail(ctx, GLS_ERR_ARG, "balance: `b -s` is not batched");
     const int32_t n = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs;
     const uint32_t narrowMax = (uint32_t)tuning_for(ctx, n).balance_narrow_max;
     hipStream_t st = ctx->stream;
@@ -818,6 +879,9 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
         }
     }
     const uint32_t batchLimit = tagged ? 0xFFFFFFFFu : (uint32_t)BAL_BATCH;
+    // the persistent workgroup takes a whole level as one batch, so a level
+    // goes there only if it is not wider than a batch
+    const uint32_t narrowCap = std::min(narrowMax, batchLimit);
     // cover lengths: the workspace width, and the steps a batch needs (a step
     // takes two inputs of every unfinished cover and gives back at most one),
     // reduced on the device
@@ -835,7 +899,7 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
     uint32_t nBatches = 0;
     for (uint32_t l = 0; l < levels; l++) {
         const uint32_t cnt = starts[l + 1] - starts[l];
-        if (cnt <= narrowMax) continue;
+        if (cnt <= narrowCap) continue;
         batchOff[l] = nBatches;
         nBatches += (cnt + std::min(cnt, batchLimit) - 1) / std::min(cnt, batchLimit);
     }
@@ -887,12 +951,12 @@ int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *in
         tagTab = Tables{nullptr, nullptr, 0, tk, tv, cap - 1};
     }
     RecArgs ra{tabs, nPIs, rootsSorted, coverOff, coverLen, pool, newLit, arr, alv, alen, maxLen, mask, skey,
-               sval, slev, &ctr[4], &ctr[2], tagged ? tagIn : nullptr, tagTab};
+               sval, slev, &ctr[4], &ctr[2], tagged ? tagIn : nullptr, tagTab, sortDecId == 0 ? 1u : 0u};
     int32_t l = (int32_t)levels - 1;
     while (l >= 0) {
         // a run of narrow levels goes to the persistent workgroup
         int32_t e = l;
-        while (e >= 0 && starts[e + 1] - starts[e] <= narrowMax) e--;
+        while (e >= 0 && starts[e + 1] - starts[e] <= narrowCap) e--;
         if (e < l) {
             k_rec_narrow<<<1, BAL_WG, 0, st>>>(ra, d_starts, l, e + 1);
             GLS_HIP(ctx, hipGetLastError());

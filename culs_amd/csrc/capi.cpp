@@ -485,13 +485,13 @@ extern "C" int gls_rewrite(gls_aig *a, int32_t use_zeros, int32_t flags) {
 extern "C" int gls_balance(gls_aig *a, int32_t sort_dec_id, int32_t update_level) {
     if (!a) return GLS_ERR_ARG;
     gls_ctx *ctx = a->ctx;
-    if (sort_dec_id != 1)
-        return gls_fail(ctx, GLS_ERR_ARG, "balance: only sort_dec_id = 1 (level, then literal) is supported");
+    if (sort_dec_id != 0 && sort_dec_id != 1)
+        return gls_fail(ctx, GLS_ERR_ARG, "balance: sort_dec_id must be 1 (`b`) or 0 (`b -s`)");
     TuneScope tune(ctx, a->nObjs);
     GLS_TRY(aig_ensure_device(a));
     DevGraph out;
     gls_op_info info = {};
-    GLS_TRY(dev_balance(ctx, a->dev, out, &info));
+    GLS_TRY(dev_balance(ctx, a->dev, out, &info, nullptr, nullptr, nullptr, sort_dec_id));
     adopt(a, out, -1);
     a->dev_level_valid = false;   // levels are not computed by the balance itself
     a->strashed = false;

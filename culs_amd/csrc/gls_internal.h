@@ -169,8 +169,10 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
 // be rebuilt in other batches than alone.  splitTags (optional) then lists
 // those tags (empty when there are more than 1024 of them).
 #define GLS_ERR_BATCH_SPLIT (-100)
+// sortDecId: 1 = `b` (cover inputs by level, then literal), 0 = `b -s` (by
+// level, ties in thrust's shuffle order; not with tags)
 int dev_balance(gls_ctx *ctx, const DevGraph &in, DevGraph &out, gls_op_info *info, const uint32_t *tagIn = nullptr,
-                uint32_t **tagOut = nullptr, std::vector<uint32_t> *splitTags = nullptr);
+                uint32_t **tagOut = nullptr, std::vector<uint32_t> *splitTags = nullptr, int sortDecId = 1);
 int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_t> &levelStart,
                        uint32_t *d_cuts, int32_t *d_nref, int mode, const uint8_t *sigBits = nullptr,
                        int *ranMode = nullptr);

@@ -127,12 +127,12 @@ int rewriteHandler(Shell &sh, const std::vector<std::string> &v) {
 int balanceHandler(Shell &sh, const std::vector<std::string> &v) {
     if (!flags_ok(v, "sv")) return 1;
     if (!need_aig(sh, "balance")) return 0;
-    if (has_flag(v, "-s")) {
-        printf("b -s: only the level-then-literal input order is available; ignored\n");
-        return 0;
-    }
+    // -s: ties of equal level in thrust's shuffle order instead of by id
+    // (command_manager.cpp:61-74)
+    const int sortDecId = has_flag(v, "-s") ? 0 : 1;
+    if (!sortDecId) printf("** balance without using id as tie break when sorting.\n");
     // balance, then strash to update levels (command_manager.cpp:77)
-    int rc = timed(sh, [&] { return gls_balance(sh.aig, 1, 1); });
+    int rc = timed(sh, [&] { return gls_balance(sh.aig, sortDecId, 1); });
     if (rc != GLS_OK) report_error(sh, "balance", rc);
     return 0;
 }

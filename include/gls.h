@@ -187,8 +187,11 @@ int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing
  * existing nodes.  Replaces AIGMan::balance (aig_manager.cu:682-737) and
- * balancePerformV2 (src/algorithms/balance.cu:912-1382).  sort_dec_id must be
- * 1 (inputs ordered by level, then literal; `b -s` is not supported).
+ * balancePerformV2 (src/algorithms/balance.cu:912-1382).  sort_dec_id 1
+ * (`b`): a cover's inputs ordered by level, then literal; 0 (`b -s`): by
+ * level, equal levels in the order of thrust::shuffle (thrust::seq,
+ * default_random_engine seeded with the cover's index in its batch,
+ * balance.cu:301, 404-411), thrust's algorithm as in rocThrust.
  * update_level = 1 strashes afterwards as the `b` command does
  * (command_manager.cpp:77); with 0 the result keeps its balance numbering
  * and its depth is unknown (levels = -1), as after `gb`. */

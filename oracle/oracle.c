@@ -382,8 +382,53 @@ static int64_t bal_lookup(const Map *tab, uint32_t a, uint32_t b) {
     return v < 0 ? -1 : 2 * (int64_t)v;
 }
 
+/* thrust::shuffle(thrust::seq, ..., default_random_engine(seed)) as a
+ * permutation: position k of the shuffled sequence holds input element
+ * perm[k].  The `b -s` tie-break shuffles each cover's inputs this way
+ * (balance.cu:404-411).  The algorithm is thrust's own (a third-party
+ * dependency of the reference, restated from rocThrust's
+ * thrust/system/detail/generic/shuffle.inl as shipped with ROCm 7.2): a
+ * 24-round Feistel bijection over the next power of two (at least 16), keys
+ * drawn from minstd_rand (x <- 48271 x mod 2^31-1, seed 0 taken as 1), then
+ * the indices below len kept in order. */
+void orc_thrust_shuffle_perm(uint32_t len, uint32_t seed, uint32_t *perm) {
+    uint32_t x = seed % 2147483647u;
+    if (x == 0) x = 1;
+    uint64_t bits = 4;
+    if (len > 16) {
+        bits = 0;
+        for (uint64_t m = (uint64_t)len - 1; m; m >>= 1) bits++;
+    }
+    const uint64_t lb = bits / 2, rb = bits - lb;
+    const uint64_t lmask = (1ull << lb) - 1, rmask = (1ull << rb) - 1;
+    uint32_t key[24];
+    for (int i = 0; i < 24; i++) {
+        x = (uint32_t)(((uint64_t)x * 48271u) % 2147483647u);
+        key[i] = x;
+    }
+    uint32_t k = 0;
+    for (uint64_t j = 0; j < (1ull << bits); j++) {
+        uint32_t s0 = (uint32_t)(j >> rb), s1 = (uint32_t)(j & rmask);
+        for (int r = 0; r < 24; r++) {
+            const uint64_t prod = 0xD2B74407B1CE6E93ull * (uint64_t)s0;
+            const uint32_t hi = (uint32_t)(prod >> 32);
+            uint32_t lo = (uint32_t)prod;
+            lo = (lo << (rb - lb)) | (s1 >> lb);
+            s0 = (uint32_t)(((hi ^ key[r]) ^ s1) & lmask);
+            s1 = (uint32_t)(lo & rmask);
+        }
+        const uint64_t out = ((uint64_t)s0 << rb) | s1;
+        if (out < len) perm[k++] = (uint32_t)out;
+    }
+}
+
 int orc_balance(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
                 int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos) {
+    return orc_balance_ex(n, nPIs, f0, f1, nPOs, pos, cap, of0, of1, opos, 1);
+}
+
+int orc_balance_ex(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                   int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos, int sortDecId) {
     int32_t *nFo = calloc((size_t)n, sizeof(int32_t));
     for (int v = nPIs + 1; v < n; v++) { nFo[ID(f0[v])]++; nFo[ID(f1[v])]++; }
     for (int i = 0; i < nPOs; i++) nFo[ID(pos[i])]++;
@@ -451,18 +496,26 @@ int orc_balance(int n, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPO
                 int32_t *A = arr + (size_t)c * (BAL_MAX_COVER + 64);
                 uint32_t *LV = alv + (size_t)c * (BAL_MAX_COVER + 64);
                 alen[c] = cov[v].len;
+                /* sortDecId = 0 (`b -s`): the inputs in the order thrust's
+                 * shuffle seeded with the cover's index in its batch puts them
+                 * (balance.cu:301, 404-411) */
+                uint32_t perm[BAL_MAX_COVER + 64];
+                if (!sortDecId && cov[v].len > 0) orc_thrust_shuffle_perm((uint32_t)cov[v].len, (uint32_t)c, perm);
                 for (int i = 0; i < cov[v].len; i++) {
-                    uint32_t lit = (uint32_t)cov[v].lit[i];
+                    uint32_t lit = (uint32_t)cov[v].lit[sortDecId ? (uint32_t)i : perm[i]];
                     int id = (int)ID(lit);
                     uint32_t nl = id <= nPIs ? lit : (newLit[id] ^ CMP(lit));
                     A[i] = (int32_t)nl;
                     LV[i] = bal_level(&lev, nl, nPIs);
                 }
-                /* level descending, then the reference's literal value descending (insertion sort) */
+                /* level descending, then (sortDecId = 1) the reference's
+                 * literal value descending; insertion sort, so that equal
+                 * keys keep their order as thrust's sequential sort does */
                 for (int i = 1; i < alen[c]; i++) {
                     int32_t x = A[i]; uint32_t xl = LV[i];
                     int j = i - 1;
-                    while (j >= 0 && (LV[j] < xl || (LV[j] == xl && culs_val((uint32_t)A[j]) < culs_val((uint32_t)x)))) {
+                    while (j >= 0 && (LV[j] < xl || (sortDecId && LV[j] == xl &&
+                                                     culs_val((uint32_t)A[j]) < culs_val((uint32_t)x)))) {
                         A[j + 1] = A[j]; LV[j + 1] = LV[j]; j--;
                     }
                     A[j + 1] = x; LV[j + 1] = xl;

@@ -38,6 +38,14 @@ int orc_strash_cpu(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, 
  * -2 if it would exceed cap. */
 int orc_balance(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
                 int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos);
+/* The same with the reference's sortDecId: 1 = `b` (level, then literal),
+ * 0 = `b -s` (level, ties in thrust's shuffle order, seeded by the cover's
+ * index in its batch). */
+int orc_balance_ex(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs, const uint32_t *pos,
+                   int cap, uint32_t *of0, uint32_t *of1, uint32_t *opos, int sortDecId);
+/* thrust::shuffle(thrust::seq, first, first + len, default_random_engine(seed))
+ * as a permutation: output position k holds input element perm[k]. */
+void orc_thrust_shuffle_perm(uint32_t len, uint32_t seed, uint32_t *perm);
 
 /* Level-major renumbering (ties by id) of an AIG whose ids are topological. */
 int orc_reorder(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,

@@ -100,17 +100,24 @@ def strash_cpu(nPIs, f0, f1, pos):
     return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy()), ol[:r].copy(), d.value
 
 
-def balance(nPIs, f0, f1, pos):
-    """`b` without its closing strash (sortDecId = 1)."""
+def balance(nPIs, f0, f1, pos, sort_dec_id=1):
+    """`b` (sort_dec_id 1) or `b -s` (0) without its closing strash."""
     f0, f1, pos = u32(f0), u32(f1), u32(pos)
     n = len(f0)
     cap = 3 * n + 64
     o0, o1 = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
     op = np.zeros(max(len(pos), 1), np.uint32)
-    r = lib().orc_balance(n, nPIs, P(f0), P(f1), len(pos), P(pos), cap, P(o0), P(o1), P(op))
+    r = lib().orc_balance_ex(n, nPIs, P(f0), P(f1), len(pos), P(pos), cap, P(o0), P(o1), P(op), int(sort_dec_id))
     if r < 0:
         raise RuntimeError("orc_balance failed %d" % r)
     return (nPIs, o0[:r].copy(), o1[:r].copy(), op[:len(pos)].copy())
+
+
+def thrust_shuffle_perm(length, seed):
+    """Output position k of thrust's shuffle holds input element perm[k]."""
+    perm = np.zeros(max(length, 1), np.uint32)
+    lib().orc_thrust_shuffle_perm(ctypes.c_uint32(length), ctypes.c_uint32(seed), P(perm))
+    return perm[:length]
 
 
 def reorder(nPIs, f0, f1, pos):

@@ -103,3 +103,55 @@ def test_balance_keeps_function_and_does_not_deepen(host_ctx, name, gen):
     # deterministic
     b2 = O.balance(*s)
     assert all(np.array_equal(x, y) for x, y in zip(b[1:], b2[1:]))
+
+
+# ---- `b -s` (sortDecId = 0): equal levels in thrust's shuffle order --------
+
+def golden_shuffles():
+    """tests/golden/thrust_shuffle.txt: permutations written by rocThrust's
+    thrust::shuffle(thrust::seq, ..., default_random_engine(seed)) through
+    tests/golden/gen_thrust_shuffle.cpp ("len seed p0 p1 ...")."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "thrust_shuffle.txt")
+    out = {}
+    with open(path) as f:
+        for line in f:
+            v = [int(x) for x in line.split()]
+            out[(v[0], v[1])] = v[2:]
+    return out
+
+
+def test_shuffle_restatement_matches_rocthrust():
+    g = golden_shuffles()
+    assert len(g) == 184
+    for (n, seed), want in g.items():
+        assert list(O.thrust_shuffle_perm(n, seed)) == want, (n, seed)
+
+
+def test_shuffle_order_known_answer():
+    # The chain's only cover (index 0 in its batch) gathers its inputs in
+    # DFS order, fanin 0 first: [d, c, a, b] (lits 8, 6, 2, 4).  The shuffle
+    # seeded with 0 is [2, 1, 0, 3] (golden line "4 0 2 1 0 3"), so the
+    # inputs become [a, c, d, b], all on level 0, which the stable level sort
+    # keeps.  Popping the last two each step: b & d first (id 5), then a & c
+    # (id 6), then the two (id 7).
+    assert golden_shuffles()[(4, 0)] == [2, 1, 0, 3]
+    nPIs, f0, f1, pos = O.balance(*chain_graph(), sort_dec_id=0)
+    assert list(f0[5:]) == [4, 2, 10] and list(f1[5:]) == [8, 6, 12]
+    assert list(pos) == [14]
+    # `b` pairs by literal instead: a & b, c & d
+    nPIs, f0, f1, pos = O.balance(*chain_graph(), sort_dec_id=1)
+    assert list(f0[5:]) == [2, 6, 10] and list(f1[5:]) == [4, 8, 12]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_shuffle_balance_keeps_function(host_ctx, seed):
+    # random AIGs: covers of several inputs on one level, where the tie order
+    # decides the pairing (multipliers' covers mostly have two inputs)
+    g = host_ctx.gen_random(64, 5000, 32, 1000, seed).arrays()
+    s, _ = O.strash(*g)
+    a = O.balance(*s, sort_dec_id=0)
+    assert O.sim_cmp(s, a) == -1
+    # the tie order matters on such a graph: `b` and `b -s` differ
+    b = O.balance(*s, sort_dec_id=1)
+    assert not all(np.array_equal(x, y) for x, y in zip(a[1:], b[1:]))

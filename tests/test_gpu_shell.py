@@ -56,3 +56,16 @@ def test_shell_cpu_strash_then_rw(gpu_ctx, tmp_path):
     a.strash(cpu=True)
     a.rewrite()
     assert same(gpu_ctx.read_aiger(out).arrays(), a.arrays())
+
+
+def test_shell_balance_shuffle(gpu_ctx, tmp_path):
+    """`b -s` runs the balance with thrust's shuffle tie order, as the
+    reference's handler does (command_manager.cpp:61-74)."""
+    src, out = str(tmp_path / "in.aig"), str(tmp_path / "out.aig")
+    a = gpu_ctx.gen_random(64, 20000, 32, 1000, 5)
+    a.write_aiger(src)
+    so = run("read %s; st; b -s; write %s" % (src, out))
+    assert "without using id as tie break" in so
+    a.strash()
+    a.balance(shuffle=True)
+    assert same(gpu_ctx.read_aiger(out).arrays(), a.arrays())
