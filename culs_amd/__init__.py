@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libgls.so")
 GLS_LIT_AIGER = 0
 GLS_LIT_CULS = 1
 GLS_ST_CPU = 1
+GLS_ST_WAVES = 2
+GLS_ST_LEVELS = 4
 GLS_RW_COOP_CUTS = 2
 GLS_RW_DATAFLOW_CUTS = 4
 GLS_RW_XCD_CUTS = 8
@@ -338,9 +340,11 @@ class Aig:
     def write_aiger(self, path):
         self.ctx.check(lib().gls_aig_write_aiger(self._h, path.encode()), "gls_aig_write_aiger")
 
-    def strash(self, cpu=False):
-        """Device strash; cpu=True is `st -c` (host strash without trivial folding)."""
-        self.ctx.check(lib().gls_strash(self._h, GLS_ST_CPU if cpu else 0), "gls_strash")
+    def strash(self, cpu=False, schedule=0):
+        """Device strash; cpu=True is `st -c` (host strash without trivial
+        folding); schedule GLS_ST_WAVES / GLS_ST_LEVELS forces one of the two
+        device schedules (same result)."""
+        self.ctx.check(lib().gls_strash(self._h, GLS_ST_CPU if cpu else int(schedule)), "gls_strash")
         return self
 
     def rewrite(self, use_zeros=False, cut_mode=0, full_strash=False):

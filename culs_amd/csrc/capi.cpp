@@ -444,7 +444,8 @@ static int strash_cpu(gls_aig *a) {
 
 extern "C" int gls_strash(gls_aig *a, int32_t flags) {
     if (!a) return GLS_ERR_ARG;
-    if (flags & ~GLS_ST_CPU) return gls_fail(a->ctx, GLS_ERR_ARG, "gls_strash: unknown flags 0x%x", flags);
+    if (flags & ~(GLS_ST_CPU | GLS_ST_WAVES | GLS_ST_LEVELS) || (flags & GLS_ST_WAVES && flags & GLS_ST_LEVELS))
+        return gls_fail(a->ctx, GLS_ERR_ARG, "gls_strash: unknown flags 0x%x", flags);
     if (flags & GLS_ST_CPU) return strash_cpu(a);
     gls_ctx *ctx = a->ctx;
     TuneScope tune(ctx, a->nObjs);
@@ -453,7 +454,17 @@ extern "C" int gls_strash(gls_aig *a, int32_t flags) {
     DevGraph out;
     int32_t depth = 0;
     gls_op_info info = {};
-    GLS_TRY(dev_strash(ctx, a->dev, out, &depth, &info));
+    // large graphs: the whole-graph mode of the incremental strash (levels by
+    // one sweep over id order, every node hashed in rounds), which gives way
+    // to the wave sweep for wide graphs and ids that are not topological
+    constexpr int32_t kLevelStrashMin = 1 << 18;
+    const bool levels = (flags & GLS_ST_LEVELS) || (!(flags & GLS_ST_WAVES) && a->nObjs >= kLevelStrashMin);
+    if (levels) {
+        const RwDelta whole{0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+        GLS_TRY(dev_strash_rw(ctx, a->dev, whole, out, &depth, &info));
+    } else {
+        GLS_TRY(dev_strash(ctx, a->dev, out, &depth, &info));
+    }
     GLS_HIP(ctx, hipEventRecord(ctx->ev[15], ctx->stream));
     GLS_HIP(ctx, hipEventSynchronize(ctx->ev[15]));
     float ms = 0;

@@ -1335,6 +1335,7 @@ static int run_sweep(gls_ctx *ctx, SweepCtl *d_ctl, uint32_t first, F launch, Sw
     return GLS_OK;
 }
 
+// tagDepth[t] = the largest wave of a node tagged t (dangling nodes have wave 0)
 __global__ void __launch_bounds__(kBlock) k_tag_depth(int32_t n, int32_t nPIs, const uint32_t *__restrict__ wave,
                                                       const uint32_t *__restrict__ tag, uint32_t *tagDepth) {
     __shared__ uint32_t s_bins[TAG_BINS];
@@ -1343,7 +1344,7 @@ __global__ void __launch_bounds__(kBlock) k_tag_depth(int32_t n, int32_t nPIs, c
     int64_t b, e;
     tag_chunk(nPIs + 1, n, b, e);
     for (int64_t v = b + threadIdx.x; v < e; v += blockDim.x)
-        if (wave[v]) tb.max(tagDepth, tag[v], wave[v]);   // dangling nodes keep wave 0
+        if (wave[v]) tb.max(tagDepth, tag[v], wave[v]);
     tb.flush_max(tagDepth, 0xFFFFFFFFu);
 }
 
@@ -1546,7 +1547,10 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
                   gls_op_info *info, const uint32_t *tagIn, uint32_t **tagOut, uint32_t *tagDepth) {
     const int32_t N = in.nObjs, nPIs = in.nPIs, nPOs = in.nPOs, n = d.n;
     hipStream_t st = ctx->stream;
-    if (!d.g0 || !d.g1 || !d.glevel || !d.tOff || !d.tEnt || n > N)
+    // n == 0: a whole graph, every live node dirty, its levels by one sweep
+    // over id order (ids must be topological, as AIGER's are)
+    const bool full = n == 0;
+    if (!full && (!d.g0 || !d.g1 || !d.glevel || !d.tOff || !d.tEnt || n > N))
         return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: the rewrite's input graph is not available");
     auto fallback = [&](const char *why) {
         (void)why;
@@ -1558,7 +1562,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
     // the two of them; the wave sweep costs about 2.6 us per level plus its
     // per-node work.  On wide graphs (many objects per level, as in a batched
     // arena) the wave sweep is the faster one.
-    if ((int64_t)N > (int64_t)tuning_for(ctx, d.n).rw_sweep_width * (int64_t)std::max(d.depth, 1)) {
+    if (!full && (int64_t)N > (int64_t)tuning_for(ctx, d.n).rw_sweep_width * (int64_t)std::max(d.depth, 1)) {
         const int rc = dev_strash(ctx, in, out, depthOut, info, tagIn, tagOut, tagDepth);
         if (info) info->final_strash = 0;
         return rc;
@@ -1599,7 +1603,24 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
             }, &h_ctl, &rounds));
         }
         // 2. a topological order (keys relaxed from the input levels, then a
-        //    stable sort), and the waves by one chunked sweep over it
+        //    stable sort), and the waves by one chunked sweep over it; for a
+        //    whole graph the ids themselves
+        uint32_t misc[16] = {};
+        bool sweepIn = true;
+        if (full) {
+            k_iota_u32<<<gN, kBlock, 0, st>>>(N, posOf);
+            k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
+            k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)N, fa, fb, levPos, &d_misc[0], &d_misc[1]);
+            GLS_HIP(ctx, hipGetLastError());
+            GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
+            if (misc[0]) return fallback("ids not topological");
+            // wide graphs are faster by the grid-wide wave sweep
+            if ((int64_t)N > (int64_t)tuning_for(ctx, N).rw_sweep_width * (int64_t)std::max<uint32_t>(misc[1], 1u)) {
+                const int rc = dev_strash(ctx, in, out, depthOut, info, tagIn, tagOut, tagDepth);
+                if (info) info->final_strash = 0;
+                return rc;
+            }
+        } else {
         k_rw_key_init<<<gN, kBlock, 0, st>>>(n, N, d.glevel, fa);
         uint32_t keyRounds = 0;
         for (;; ++keyRounds) {
@@ -1625,9 +1646,8 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         // level sweep over the positions
         k_rw_key_loose<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, fa, &d_misc[9], &d_misc[10]);
         GLS_HIP(ctx, hipGetLastError());
-        uint32_t misc[16] = {};
         GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
-        const bool sweepIn = misc[9] != 0u;
+        sweepIn = misc[9] != 0u;
         if (sweepIn) {
             k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
             GLS_HIP(ctx, hipGetLastError());
@@ -1641,6 +1661,7 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
             k_rw_levpos<<<gN, kBlock, 0, st>>>(N, nPIs, dead, fa, posOf, levPos);
             GLS_HIP(ctx, hipGetLastError());
             misc[1] = misc[10];
+        }
         }
         const int32_t depth = (int32_t)misc[1];
         k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD,
@@ -1740,6 +1761,9 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
                 return gls_fail(ctx, GLS_ERR_INTERNAL, "strash: output not topological");
             }
         }
+        // the depth reported is the number of waves of the input (its live
+        // nodes' largest level), as the reference's strash counts it, not
+        // the output's largest level
         if (depthOut) *depthOut = depth;
         if (info) {
             info->levels_enumerated = depth;

@@ -150,8 +150,16 @@ int gls_aig_from_net(gls_ctx *ctx, const gls_net *net, gls_aig **out);
  * flags = GLS_ST_CPU selects the host strash of `st -c`
  * (AIGMan::strash(fCPU=true), strash.cu:322-466): dangling removal and
  * id-order hashing without trivial-case folding, on the host (also in a
- * host-only context); levels = the largest node delay, -1 if no AND remains. */
+ * host-only context); levels = the largest node delay, -1 if no AND remains.
+ * The device strash has two schedules with the same result: the wave sweep
+ * (frontier by frontier) and, for graphs of at least 2^18 objects whose ids
+ * are topological, one chunked level sweep over id order followed by hashing
+ * rounds over all nodes at once (it gives way to the wave sweep when the
+ * graph turns out wide or its ids are not topological).  GLS_ST_WAVES and
+ * GLS_ST_LEVELS force either at any size (for measurement and testing). */
 #define GLS_ST_CPU 1
+#define GLS_ST_WAVES 2
+#define GLS_ST_LEVELS 4
 int gls_strash(gls_aig *aig, int32_t flags);
 
 /* One DAG-aware rewriting pass with 4-input cuts (rw; use_zeros=1 is rw -z /

@@ -45,14 +45,20 @@ def same(a, b):
 
 @pytest.mark.parametrize("name,spec", CASES, ids=[c[0] for c in CASES])
 def test_strash_parity(gpu_ctx, name, spec):
-    log("strash parity: %s" % name)
+    """Both device schedules (the wave sweep, and the level sweep with hashing
+    rounds over all nodes) and the size-based default give the oracle's AIG
+    and depth."""
+    import culs_amd
     a = make(gpu_ctx, spec)
     g = a.arrays()
-    a.strash()
-    got = a.arrays()
     want, depth = O.strash(*g)
-    assert same(got, want), "GPU strash differs from the oracle"
-    assert a.stats().levels == depth
+    for sched in (0, culs_amd.GLS_ST_WAVES, culs_amd.GLS_ST_LEVELS):
+        log("strash parity: %s, schedule %d" % (name, sched))
+        b = a.copy()
+        b.strash(schedule=sched)
+        got = b.arrays()
+        assert same(got, want), "GPU strash differs from the oracle (schedule %d)" % sched
+        assert b.stats().levels == depth, "depth under schedule %d" % sched
     assert O.sim_cmp(g, got) == -1
 
 
