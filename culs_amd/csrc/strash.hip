@@ -37,13 +37,12 @@ __global__ void k_count_refs(int32_t n, int32_t nPIs, const uint32_t *__restrict
     for (int64_t i = tid; i < nPOs; i += stride) atomicAdd(&ref[LID(pos[i])], 1u);
 }
 
-__global__ void k_seed_zero(int32_t n, int32_t nPIs, const uint32_t *__restrict__ cnt, uint32_t *counter,
-                            uint32_t *order, const uint8_t *__restrict__ dead) {
+// flag[v] = 1 for a live AND whose count is 0 (the first frontier of the
+// dangling cascade or of the wave sweep); dead may be null
+__global__ void k_zero_flags(int32_t n, int32_t nPIs, const uint32_t *__restrict__ cnt,
+                             const uint8_t *__restrict__ dead, uint8_t *flag) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = nPIs + 1 + tid; v < n; v += stride) {
-        if (dead && dead[v]) continue;
-        if (cnt[v] == 0) order[atomicAdd(counter, 1u)] = (uint32_t)v;
-    }
+    for (int64_t v = tid; v < n; v += stride) flag[v] = (v > nPIs && cnt[v] == 0 && !(dead && dead[v])) ? 1u : 0u;
 }
 
 // One round of the dangling cascade (strash.cu:175-188 as a frontier).
@@ -1060,6 +1059,30 @@ __global__ void k_rw_pos(int32_t N, const uint32_t *__restrict__ order, uint32_t
     for (int64_t p = tid; p < N; p += stride) posOf[order[p]] = (uint32_t)p;
 }
 
+// Positions of a whole graph's level sweep: the constant, the PIs and the
+// live ANDs keep one each, in id order (keep[v] = 1), dangling ANDs none.
+__global__ void k_keep_flags(int32_t N, int32_t nPIs, const uint8_t *__restrict__ dead, uint32_t *keep) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) keep[v] = (v <= nPIs || !dead[v]) ? 1u : 0u;
+}
+
+// Fanin positions at those positions (posOf: exclusive scan of keep); the
+// constant and the PIs have none.
+__global__ void k_full_fanpos(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
+                              const uint32_t *__restrict__ f1, const uint8_t *__restrict__ dead,
+                              const uint32_t *__restrict__ posOf, uint32_t *fa, uint32_t *fb) {
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = tid; v < N; v += stride) {
+        if (v <= nPIs) {
+            fa[posOf[v]] = NOFAN;
+            fb[posOf[v]] = NOFAN;
+        } else if (!dead[v]) {
+            fa[posOf[v]] = posOf[LID(f0[v])];   // fanins of a live node are live
+            fb[posOf[v]] = posOf[LID(f1[v])];
+        }
+    }
+}
+
 // Fanin positions of every live AND at its sweep position; PIs, the constant
 // and dangling nodes have none.
 __global__ void k_rw_fanpos(int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0, const uint32_t *__restrict__ f1,
@@ -1076,13 +1099,15 @@ __global__ void k_rw_fanpos(int32_t N, int32_t nPIs, const uint32_t *__restrict_
 }
 
 // Wave of every object (0 for PIs, the constant and dangling nodes), the
-// initial representative literal (itself), and the first dirty set: live new
-// nodes and live old nodes with a redirected fanin.
+// initial representative literal (itself), and the first dirty set as flags
+// (inD): live new nodes and live old nodes with a redirected fanin.  (The list
+// is a compaction of the flags: appends to one counter serialise on its
+// address, about 1.8 ms for the 156k wavefronts of a 10M-node graph.)
 __global__ void k_rw_seed(int32_t n, int32_t N, int32_t nPIs, const uint32_t *__restrict__ f0,
                           const uint32_t *__restrict__ f1, const uint32_t *__restrict__ g0,
                           const uint32_t *__restrict__ g1, const uint8_t *__restrict__ dead,
                           const uint32_t *__restrict__ posOf, const uint32_t *__restrict__ levPos, uint32_t *wave,
-                          uint32_t *rep, uint32_t *inD, uint32_t *dlist, uint32_t *dcount) {
+                          uint32_t *rep, uint32_t *inD) {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v = tid; v < N; v += stride) {
         const bool live = v > nPIs && !dead[v];
@@ -1090,7 +1115,6 @@ __global__ void k_rw_seed(int32_t n, int32_t N, int32_t nPIs, const uint32_t *__
         rep[v] = 2u * (uint32_t)v;
         const bool d = live && (v >= n || f0[v] != g0[v] || f1[v] != g1[v]);
         inD[v] = d ? 1u : 0u;
-        if (d) dlist[atomicAdd(dcount, 1u)] = (uint32_t)v;
     }
 }
 
@@ -1303,6 +1327,20 @@ struct Tmp {
     }
 };
 
+// order[0 .. *count) = the ids v with flag[v] set, ascending (a compaction:
+// appends to one counter serialise on its address)
+static int select_ids(gls_ctx *ctx, Tmp &t, int32_t n, const uint8_t *flag, uint32_t *order, uint32_t *count) {
+    hipStream_t st = ctx->stream;
+    rocprim::counting_iterator<uint32_t> ids(0u);
+    size_t tb = 0;
+    GLS_HIP(ctx, rocprim::select(nullptr, tb, ids, flag, order, count, (size_t)n, st));
+    void *tmp = t.get<uint8_t>(std::max<size_t>(tb, 1));
+    if (!tmp) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: list storage");
+    GLS_HIP(ctx, rocprim::select(tmp, tb, ids, flag, order, count, (size_t)n, st));
+    return GLS_OK;
+}
+
+
 // Grid of a sweep round: sized from the frontier seen at the start of a batch,
 // never below 8 workgroups so that a growing frontier is not serialised.
 int sweep_grid(uint32_t hint) {
@@ -1392,9 +1430,12 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     uint32_t rounds = 0;
 
     // 1. fanout counts and dangling cascade
+    uint8_t *zflag = t.get<uint8_t>((size_t)n);
+    if (!zflag) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: frontier flags");
     k_count_refs<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, nPOs, in.pos, ref);
-    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, ref, &ctl->curSize, order, nullptr);
+    k_zero_flags<<<gN, kBlock, 0, st>>>(n, nPIs, ref, nullptr, zflag);
     GLS_HIP(ctx, hipGetLastError());
+    GLS_TRY(select_ids(ctx, t, n, zflag, order, &ctl->curSize));
     GLS_HIP(ctx, hipMemcpyAsync(&h_ctl, ctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, st));
     GLS_HIP(ctx, hipStreamSynchronize(st));
     if (h_ctl.curSize > 0) {
@@ -1417,7 +1458,8 @@ int dev_strash(gls_ctx *ctx, const DevGraph &in, DevGraph &out, int32_t *depthOu
     GLS_HIP(ctx, rocprim::exclusive_scan(tmp, tb, cnt, adjOff, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
     GLS_HIP(ctx, hipMemcpyAsync(cursor, adjOff, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
     k_kahn_fill<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, cnt, cursor, adj, adjY);
-    k_seed_zero<<<gN, kBlock, 0, st>>>(n, nPIs, pending, &fcnt[1], order, dead);
+    k_zero_flags<<<gN, kBlock, 0, st>>>(n, nPIs, pending, dead, zflag);
+    GLS_TRY(select_ids(ctx, t, n, zflag, order, &fcnt[1]));
     unsigned long long *d_probes = reinterpret_cast<unsigned long long *>(&d_misc[4]);
     k_preinsert<<<gN, kBlock, 0, st>>>(n, nPIs, in.f0, in.f1, dead, ht, cap - 1, slotTmp, &d_misc[1], d_probes);
     k_canon<<<gN, kBlock, 0, st>>>(n, nPIs, dead, ht, slotTmp, locOf);
@@ -1591,10 +1633,12 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         const int gN = grid_for(N);
         SweepCtl h_ctl{};
         uint32_t rounds = 0;
-        // 1. dangling cascade
+        // 1. dangling cascade (its first frontier, the ANDs without fanout, by
+        //    a compaction of flags)
         k_count_refs<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, nPOs, in.pos, ref);
-        k_seed_zero<<<gN, kBlock, 0, st>>>(N, nPIs, ref, &ctl->curSize, order, nullptr);
+        k_zero_flags<<<gN, kBlock, 0, st>>>(N, nPIs, ref, nullptr, surv);
         GLS_HIP(ctx, hipGetLastError());
+        GLS_TRY(select_ids(ctx, t, N, surv, order, &ctl->curSize));
         GLS_HIP(ctx, hipMemcpyAsync(&h_ctl, ctl, sizeof(SweepCtl), hipMemcpyDeviceToHost, st));
         GLS_HIP(ctx, hipStreamSynchronize(st));
         if (h_ctl.curSize > 0) {
@@ -1608,9 +1652,21 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         uint32_t misc[16] = {};
         bool sweepIn = true;
         if (full) {
-            k_iota_u32<<<gN, kBlock, 0, st>>>(N, posOf);
-            k_rw_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb, &d_misc[0]);
-            k_level_sweep<<<1, SW_WG, 0, st>>>((uint32_t)N, fa, fb, levPos, &d_misc[0], &d_misc[1]);
+            // positions: the constant and the PIs, then the live ANDs in id
+            // order (inD is free until the dirty set is seeded)
+            k_keep_flags<<<gN, kBlock, 0, st>>>(N, nPIs, dead, inD);
+            size_t tbp = 0;
+            GLS_HIP(ctx, rocprim::exclusive_scan(nullptr, tbp, inD, posOf, 0u, (size_t)N, rocprim::plus<uint32_t>(), st));
+            void *tmpp = t.get<uint8_t>(std::max<size_t>(tbp, 1));
+            if (!tmpp) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: position scan storage");
+            GLS_HIP(ctx, rocprim::exclusive_scan(tmpp, tbp, inD, posOf, 0u, (size_t)N, rocprim::plus<uint32_t>(), st));
+            k_full_fanpos<<<gN, kBlock, 0, st>>>(N, nPIs, in.f0, in.f1, dead, posOf, fa, fb);
+            GLS_HIP(ctx, hipGetLastError());
+            uint32_t lastPos = 0, lastKeep = 0;
+            GLS_TRY(stream_copy(ctx, &lastPos, posOf + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
+            GLS_TRY(stream_copy(ctx, &lastKeep, inD + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
+            const uint32_t nPos = lastPos + lastKeep;
+            k_level_sweep<<<1, SW_WG, 0, st>>>(nPos, fa, fb, levPos, &d_misc[0], &d_misc[1]);
             GLS_HIP(ctx, hipGetLastError());
             GLS_TRY(stream_copy(ctx, misc, d_misc, sizeof(misc), hipMemcpyDeviceToHost));
             if (misc[0]) return fallback("ids not topological");
@@ -1664,9 +1720,16 @@ int dev_strash_rw(gls_ctx *ctx, const DevGraph &in, const RwDelta &d, DevGraph &
         }
         }
         const int32_t depth = (int32_t)misc[1];
-        k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD,
-                                         dlist, &d_misc[2]);
+        k_rw_seed<<<gN, kBlock, 0, st>>>(n, N, nPIs, in.f0, in.f1, d.g0, d.g1, dead, posOf, levPos, wave, rep, inD);
         GLS_HIP(ctx, hipGetLastError());
+        {
+            rocprim::counting_iterator<uint32_t> ids(0u);
+            size_t tbd = 0;
+            GLS_HIP(ctx, rocprim::select(nullptr, tbd, ids, inD, dlist, &d_misc[2], (size_t)N, st));
+            void *tmpd = t.get<uint8_t>(std::max<size_t>(tbd, 1));
+            if (!tmpd) return gls_fail(ctx, GLS_ERR_NOMEM, "strash: dirty list storage");
+            GLS_HIP(ctx, rocprim::select(tmpd, tbd, ids, inD, dlist, &d_misc[2], (size_t)N, st));
+        }
         if (tagIn && tagDepth) k_tag_depth<<<tag_grid(N), kBlock, 0, st>>>(N, nPIs, wave, tagIn, tagDepth);
         // 3. representatives of the dirty set, in rounds
         uint32_t nD = 0;
