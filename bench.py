@@ -55,7 +55,8 @@ def cut_kernel_name(mode):
     mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     return {culs_amd.GLS_RW_XCD_CUTS: "k_cuts_xcd", culs_amd.GLS_RW_LEVEL_CUTS: "k_cuts_level",
             culs_amd.GLS_RW_COOP_CUTS: "k_cuts_levels", culs_amd.GLS_RW_DATAFLOW_CUTS: "k_cuts_dataflow",
-            culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair", culs_amd.GLS_RW_PLAN_CUTS: "k_cuts_plan"}.get(mode, "k_cuts_triple")
+            culs_amd.GLS_RW_PAIR_CUTS: "k_cuts_pair", culs_amd.GLS_RW_PLAN_CUTS: "k_cuts_plan",
+            culs_amd.GLS_RW_QUAD_CUTS: "k_cuts_quad"}.get(mode, "k_cuts_triple")
 
 
 def cut_launches(mode, nlev, levels=3):
@@ -65,7 +66,8 @@ def cut_launches(mode, nlev, levels=3):
     mode &= ~culs_amd.GLS_RW_GRAPH_CUTS
     if mode in (culs_amd.GLS_RW_XCD_CUTS, culs_amd.GLS_RW_COOP_CUTS, culs_amd.GLS_RW_DATAFLOW_CUTS):
         return 1
-    per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2, culs_amd.GLS_RW_PLAN_CUTS: levels}.get(mode, 3)
+    per = {culs_amd.GLS_RW_LEVEL_CUTS: 1, culs_amd.GLS_RW_PAIR_CUTS: 2, culs_amd.GLS_RW_PLAN_CUTS: levels,
+           culs_amd.GLS_RW_QUAD_CUTS: 4}.get(mode, 3)
     return (nlev + per - 1) // per
 
 

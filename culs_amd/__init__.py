@@ -26,6 +26,7 @@ GLS_RW_PAIR_CUTS = 32
 GLS_RW_FULL_STRASH = 64
 GLS_RW_GRAPH_CUTS = 128
 GLS_RW_PLAN_CUTS = 256
+GLS_RW_QUAD_CUTS = 512
 
 
 class GlsError(RuntimeError):

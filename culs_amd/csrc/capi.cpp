@@ -558,7 +558,7 @@ extern "C" int gls_enumerate_cuts(gls_aig *a, uint32_t *cuts) { return gls_enume
 extern "C" int gls_enumerate_cuts_sched(gls_aig *a, int32_t flags, uint32_t *cuts) {
     if (!a || !cuts) return GLS_ERR_ARG;
     if (flags & ~(GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS | GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS |
-                  GLS_RW_GRAPH_CUTS | GLS_RW_PLAN_CUTS))
+                  GLS_RW_GRAPH_CUTS | GLS_RW_PLAN_CUTS | GLS_RW_QUAD_CUTS))
         return gls_fail(a->ctx, GLS_ERR_ARG, "gls_enumerate_cuts_sched: unknown flags 0x%x", flags);
     gls_ctx *ctx = a->ctx;
     GLS_TRY(aig_ensure_device(a));

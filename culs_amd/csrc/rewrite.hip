@@ -647,6 +647,90 @@ __global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_triple(int32_t s0_, int32_t
 // Entry (uint4): x = node id | stored-globally << 31 | kept-in-LDS << 30,
 //   y, z = fanin literals, w = fanin sources (8 bits each: an entry index of
 //   this task, or PLAN_GLOBAL).
+#include "cut_cone.h"
+
+// Four levels in one launch (ids [s0, s1), [s1, s2), [s2, s3), [s3, e)), one
+// workgroup of eight wavefronts per node, as k_cuts_triple with one more
+// level: a node recomputes its in-launch cone (at most 8, 4 and 2 nodes on
+// the launch's three lower levels, listed by one thread into LDS) level by
+// level, one wavefront per cone node, records in LDS, a barrier between the
+// levels; its own set goes to the global records.  One launch replaces four
+// dependent ones at the price of larger cones.
+constexpr int CUT4_BLOCK = 512;
+
+// the record of fanin x for a cone node: its LDS slot if the cone lists it,
+// else the global records (x below the launch)
+__device__ __forceinline__ const uint32_t *cone4_rec(uint32_t x, const uint32_t *s, const Cone4 &c,
+                                                     uint32_t (*s_rec)[NCUT * 5], const uint32_t *cuts) {
+    const int p = cone4_level(x, s);
+    if (p >= 0 && p <= 2) {
+        const uint32_t base = p == 0 ? 0u : p == 1 ? 8u : 12u;
+        for (uint32_t k = 0; k < c.n[p]; k++)
+            if (c.id[p][k] == x) return s_rec[base + k];
+    }
+    return cuts + (size_t)x * NCUT * 5;
+}
+
+__global__ void __launch_bounds__(CUT4_BLOCK) k_cuts_quad(int32_t s0_, int32_t s1_, int32_t s2_, int32_t s3_,
+                                                          const uint32_t *__restrict__ f0,
+                                                          const uint32_t *__restrict__ f1,
+                                                          const int32_t *__restrict__ nRef, uint32_t *cuts,
+                                                          const uint8_t *__restrict__ sigBits) {
+    __shared__ uint32_t s_rec[14][NCUT * 5];    // level 0: 0..7, level 1: 8..11, level 2: 12..13
+    __shared__ Cone4 s_cone;
+    const uint32_t s[4] = {(uint32_t)s0_, (uint32_t)s1_, (uint32_t)s2_, (uint32_t)s3_};
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t v = s[0] + blockIdx.x;
+    const int pv = cone4_level(v, s);
+    // issue priority by chain length: a top-level node's workgroup runs four
+    // dependent steps and bounds the launch
+    if (pv == 0) {                              // uniform per workgroup
+        __builtin_amdgcn_s_setprio(0);
+        if (w == 0) cut_node_global((int)v, lane, f0, f1, nRef, cuts, sigBits);
+        return;
+    }
+    if (pv == 3) __builtin_amdgcn_s_setprio(3);
+    else if (pv == 2) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(1);
+    if (threadIdx.x == 0) cone4_discover(v, s, f0, f1, s_cone);
+    __syncthreads();
+    const uint32_t n0 = s_cone.n[0], n1 = s_cone.n[1], n2 = s_cone.n[2];
+    // level 0 of the cone: fanins below the launch
+    if (w < n0) {
+        const uint32_t x = s_cone.id[0][w], g0 = s_cone.l0[0][w], g1 = s_cone.l1[0][w];
+        cut_node<REC_PLAIN>((int)x, lane, g0, g1, cuts + (size_t)LID(g0) * NCUT * 5,
+                            cuts + (size_t)LID(g1) * NCUT * 5, s_rec[w], nRef, nullptr, nullptr, sigBits);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __syncthreads();
+    // level 1
+    if (w < n1) {
+        const uint32_t x = s_cone.id[1][w], g0 = s_cone.l0[1][w], g1 = s_cone.l1[1][w];
+        cut_node<REC_PLAIN>((int)x, lane, g0, g1, cone4_rec(LID(g0), s, s_cone, s_rec, cuts),
+                            cone4_rec(LID(g1), s, s_cone, s_rec, cuts), s_rec[8 + w], nRef, nullptr, nullptr,
+                            sigBits);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __syncthreads();
+    // level 2
+    if (w < n2) {
+        const uint32_t x = s_cone.id[2][w], g0 = s_cone.l0[2][w], g1 = s_cone.l1[2][w];
+        cut_node<REC_PLAIN>((int)x, lane, g0, g1, cone4_rec(LID(g0), s, s_cone, s_rec, cuts),
+                            cone4_rec(LID(g1), s, s_cone, s_rec, cuts), s_rec[12 + w], nRef, nullptr, nullptr,
+                            sigBits);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __syncthreads();
+    // the node itself
+    if (w == 0) {
+        const uint32_t l0 = f0[v], l1 = f1[v];
+        cut_node<REC_PLAIN>((int)v, lane, l0, l1, cone4_rec(LID(l0), s, s_cone, s_rec, cuts),
+                            cone4_rec(LID(l1), s, s_cone, s_rec, cuts), cuts + (size_t)v * NCUT * 5, nRef, nullptr,
+                            nullptr, sigBits);
+    }
+}
+
 #include "cut_plan.h"
 
 // One planned task per workgroup (the launch's tasks start at `base`).
@@ -1782,10 +1866,11 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
         // launch each.
         // entries hold node ids in 30 bits
         const bool planned = mode == GLS_RW_PLAN_CUTS && n < (1 << 30);
-        const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2
+        const int step = mode == GLS_RW_LEVEL_CUTS ? 1 : mode == GLS_RW_PAIR_CUTS ? 2 : mode == GLS_RW_QUAD_CUTS ? 4
                          : planned ? std::min(std::max(tuning_for(ctx, n).cut_levels, 1), PLAN_MAX_LEVELS) : 3;
         const int32_t cutWide = tuning_for(ctx, n).cut_wide;
-        ran = planned ? GLS_RW_PLAN_CUTS : step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS : 0;
+        ran = planned ? GLS_RW_PLAN_CUTS : step == 1 ? GLS_RW_LEVEL_CUTS : step == 2 ? GLS_RW_PAIR_CUTS
+              : step == 4 ? GLS_RW_QUAD_CUTS : 0;
         struct CutLaunch { int kind; int32_t s, m, m2, e, l; };
         std::vector<CutLaunch> plan;
         plan.reserve((size_t)L / step + 1);
@@ -1863,6 +1948,10 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                     hipExtLaunchKernelGGL(k_cuts_plan, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0,
                                           (const uint4 *)heads, (const uint4 *)ents, (int32_t)tbase, (int32_t)step,
                                           (const int32_t *)d_nref, d_cuts, sigBits);
+                else if (c.kind == 4)
+                    hipExtLaunchKernelGGL(k_cuts_quad, grid, dim3(CUT4_BLOCK), 0, st, evStart, evStop, 0, c.s, c.m,
+                                          c.m2, levelStart[std::min(c.l + 3, L)], (const uint32_t *)g.f0,
+                                          (const uint32_t *)g.f1, (const int32_t *)d_nref, d_cuts, sigBits);
                 else if (c.kind == 3)
                     hipExtLaunchKernelGGL(k_cuts_triple, grid, dim3(CUT3_BLOCK), 0, st, evStart, evStop, 0, c.s, c.m,
                                           c.m2, (const uint32_t *)g.f0, (const uint32_t *)g.f1,
@@ -1877,6 +1966,9 @@ int dev_enumerate_cuts(gls_ctx *ctx, const DevGraph &g, const std::vector<int32_
                                           (const int32_t *)d_nref, d_cuts, sigBits);
             } else if (pl) {
                 k_cuts_plan<<<grid, CUT3_BLOCK, 0, st>>>(heads, ents, (int32_t)tbase, step, d_nref, d_cuts, sigBits);
+            } else if (c.kind == 4) {
+                k_cuts_quad<<<grid, CUT4_BLOCK, 0, st>>>(c.s, c.m, c.m2, levelStart[std::min(c.l + 3, L)], g.f0, g.f1,
+                                                         d_nref, d_cuts, sigBits);
             } else if (c.kind == 3) {
                 k_cuts_triple<<<grid, CUT3_BLOCK, 0, st>>>(c.s, c.m, c.m2, g.f0, g.f1, d_nref, d_cuts, sigBits);
             } else if (c.kind == 2) {
@@ -1993,7 +2085,7 @@ int dev_rewrite(gls_ctx *ctx, const DevGraph &in, int useZeros, int flags, DevGr
         GLS_TRY(dev_enumerate_cuts(ctx, g, levelStart, cuts, nref,
                                    flags & (GLS_RW_COOP_CUTS | GLS_RW_DATAFLOW_CUTS | GLS_RW_XCD_CUTS |
                                             GLS_RW_LEVEL_CUTS | GLS_RW_PAIR_CUTS | GLS_RW_GRAPH_CUTS |
-                                            GLS_RW_PLAN_CUTS),
+                                            GLS_RW_PLAN_CUTS | GLS_RW_QUAD_CUTS),
                                    sigBits, &cutRan));
         tm.mark(); // 2
         GraphView gv{g.f0, g.f1, nref, g.level, nullptr, nullptr};

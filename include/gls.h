@@ -191,6 +191,10 @@ int gls_strash(gls_aig *aig, int32_t flags);
  * recomputes, one owner each).  Currently DISABLED: refused with GLS_ERR_ARG,
  * because its kernel faulted the GPU in testing and the cause is not found. */
 #define GLS_RW_PLAN_CUTS 256
+/* GLS_RW_QUAD_CUTS: four levels per launch (k_cuts_quad: a node recomputes its
+ * in-launch cone of up to 8 + 4 + 2 nodes in its workgroup), for measurement
+ * against the default three. */
+#define GLS_RW_QUAD_CUTS 512
 int gls_rewrite(gls_aig *aig, int32_t use_zeros, int32_t flags);
 
 /* Balance (`b`): rebuild every AND cover as a tree of least depth, sharing

@@ -1,7 +1,7 @@
 """Cut-enumeration schedules on the GPU, checked against the oracle bit for bit.
 
 The default schedule covers three levels per launch (a node recomputes its
-in-launch fanin cone in its workgroup) and falls back to one launch per level
+in-launch fanin cone in its workgroup; GLS_RW_QUAD_CUTS four) and falls back to one launch per level
 for any group of three levels holding a level wider than 2048 nodes.  These
 layered AIGs place wide and narrow levels, widths on both sides of that limit
 and long runs of one-node levels at every offset within a group, so groups of
@@ -83,10 +83,12 @@ def test_cut_schedule_parity(gpu_ctx, name, spec):
             what, int(np.argmax((cuts != want_cuts).reshape(-1, 40).any(axis=1))))
 
     same_cuts(b.enumerate_cuts(), "the default schedule")
+    same_cuts(b.enumerate_cuts(culs_amd.GLS_RW_QUAD_CUTS), "four levels per launch")
     want, _ = O.rw(*s)
     outs = []
     G = culs_amd.GLS_RW_GRAPH_CUTS
-    for mode in (0, culs_amd.GLS_RW_LEVEL_CUTS, culs_amd.GLS_RW_PAIR_CUTS, G, G | culs_amd.GLS_RW_LEVEL_CUTS):
+    Q = culs_amd.GLS_RW_QUAD_CUTS
+    for mode in (0, culs_amd.GLS_RW_LEVEL_CUTS, culs_amd.GLS_RW_PAIR_CUTS, Q, G, G | culs_amd.GLS_RW_LEVEL_CUTS, G | Q):
         x = a.copy()
         x.rewrite(cut_mode=mode)
         info = x.last_op()
