@@ -650,13 +650,16 @@ __global__ void __launch_bounds__(CUT3_BLOCK) k_cuts_triple(int32_t s0_, int32_t
 #include "cut_cone.h"
 
 // Four levels in one launch (ids [s0, s1), [s1, s2), [s2, s3), [s3, e)), one
-// workgroup of eight wavefronts per node, as k_cuts_triple with one more
+// workgroup of four wavefronts per node, as k_cuts_triple with one more
 // level: a node recomputes its in-launch cone (at most 8, 4 and 2 nodes on
 // the launch's three lower levels, listed by one thread into LDS) level by
-// level, one wavefront per cone node, records in LDS, a barrier between the
-// levels; its own set goes to the global records.  One launch replaces four
-// dependent ones at the price of larger cones.
-constexpr int CUT4_BLOCK = 512;
+// level, one wavefront per cone node (the lowest level's nodes beyond four in
+// a second batch), records in LDS, a barrier between the levels; its own set
+// goes to the global records.  One launch replaces four dependent ones at the
+// price of larger cones.  (Eight wavefronts per workgroup, one per lowest-level
+// node, put two of a workgroup's own wavefronts on every SIMD and measured
+// 17.9 us per launch against 10.9 for three levels.)
+constexpr int CUT4_BLOCK = 256;   // (measured 159.5 ms per pass on the 10M AIG against 138.1 for three levels)
 
 // the record of fanin x for a cone node: its LDS slot if the cone lists it,
 // else the global records (x below the launch)
@@ -696,11 +699,16 @@ __global__ void __launch_bounds__(CUT4_BLOCK) k_cuts_quad(int32_t s0_, int32_t s
     if (threadIdx.x == 0) cone4_discover(v, s, f0, f1, s_cone);
     __syncthreads();
     const uint32_t n0 = s_cone.n[0], n1 = s_cone.n[1], n2 = s_cone.n[2];
-    // level 0 of the cone: fanins below the launch
+    // level 0 of the cone: fanins below the launch; nodes 4..7 in a second batch
     if (w < n0) {
         const uint32_t x = s_cone.id[0][w], g0 = s_cone.l0[0][w], g1 = s_cone.l1[0][w];
         cut_node<REC_PLAIN>((int)x, lane, g0, g1, cuts + (size_t)LID(g0) * NCUT * 5,
                             cuts + (size_t)LID(g1) * NCUT * 5, s_rec[w], nRef, nullptr, nullptr, sigBits);
+    }
+    if (w + 4 < n0) {
+        const uint32_t x = s_cone.id[0][w + 4], g0 = s_cone.l0[0][w + 4], g1 = s_cone.l1[0][w + 4];
+        cut_node<REC_PLAIN>((int)x, lane, g0, g1, cuts + (size_t)LID(g0) * NCUT * 5,
+                            cuts + (size_t)LID(g1) * NCUT * 5, s_rec[w + 4], nRef, nullptr, nullptr, sigBits);
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __syncthreads();

@@ -897,12 +897,16 @@ __global__ void k_map_pos(int32_t nPOs, int32_t nPIs, const uint32_t *__restrict
 // workgroup: lev[p] = 0 where fa[p] == NOFAN, else 1 + max(lev[fa[p]],
 // lev[fb[p]]), fanin positions below p.  Positions go in chunks of SW_WG, one
 // per thread.  The levels of the last SW_RING positions stay in an LDS ring
-// (one word each, SW_NOLEV until known), so a fanin at most SW_RING - SW_WG
+// (one word each, SW_NOLEV until known), so a fanin at most SW_RING - 2 SW_WG
 // positions back is read from LDS; older ones from lev in global memory.  A
 // position whose fanin lies in its own chunk waits for that slot in LDS: every
 // fanin sits at a lower position (the same or an earlier wavefront of the
 // workgroup), so each wait ends; it is bounded anyway, and a fanin at or above
-// its position sets *err instead of waiting.
+// its position sets *err instead of waiting.  Each thread clears the slot of
+// its next chunk's position during the current chunk (a slot no position of
+// the current chunk reads), so the one barrier per chunk both publishes the
+// chunk's levels and leaves the next chunk's slots cleared: no waiting
+// position can read a level left there by the position SW_RING earlier.
 constexpr int SW_WG = 1024;
 constexpr int SW_RING = 32768;              // 128 KiB of LDS
 constexpr uint32_t NOFAN = 0xFFFFFFFFu;
@@ -934,9 +938,9 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
         const uint32_t a = f.x, b = f.y;
         const bool leaf = !active || a == NOFAN;
         const bool bad = !leaf && (a >= p || b >= p);
-        // ring slots of this chunk held positions SW_RING earlier, which this
-        // chunk reads from global memory
-        const uint32_t oldest = c > (uint32_t)(SW_RING - SW_WG) ? c - (uint32_t)(SW_RING - SW_WG) : 0u;
+        // ring slots of this chunk and the next held positions SW_RING earlier,
+        // which this chunk reads from global memory
+        const uint32_t oldest = c > (uint32_t)(SW_RING - 2 * SW_WG) ? c - (uint32_t)(SW_RING - 2 * SW_WG) : 0u;
         uint32_t la = 0, lb = 0;
         bool needA = false, needB = false;
         if (!leaf && !bad) {
@@ -950,8 +954,10 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
         const uint32_t q = p + SW_WG;
         const uint32_t na = q < nPos ? fa[q] : NOFAN;
         const uint32_t nb = q < nPos ? fb[q] : NOFAN;
-        ring[p & (SW_RING - 1)] = active && leaf ? 0u : SW_NOLEV;
-        __syncthreads();
+        // a leaf publishes at once; the other slots of this chunk were cleared
+        // during the previous one (or before the first); clear the next one's
+        if (active && leaf) __hip_atomic_store(&ring[p & (SW_RING - 1)], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        ring[(p + SW_WG) & (SW_RING - 1)] = SW_NOLEV;
         uint32_t val = 0;
         // the value is published inside the loop, in the iteration that
         // completes it: a lane of the same wavefront may be waiting for it
