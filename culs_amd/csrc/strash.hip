@@ -963,15 +963,14 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
         // completes it: a lane of the same wavefront may be waiting for it
         bool done = leaf || bad;
         uint32_t spins = 0;
+        // both slots are read every round (a and b are positions below p here,
+        // so the masked slots are in the ring), so the two LDS reads are in
+        // flight together instead of one after the other
         while (!done) {
-            if (needA) {
-                const uint32_t x = sw_lds_load(&ring[a & (SW_RING - 1)]);
-                if (x != SW_NOLEV) { la = x; needA = false; }
-            }
-            if (needB) {
-                const uint32_t x = sw_lds_load(&ring[b & (SW_RING - 1)]);
-                if (x != SW_NOLEV) { lb = x; needB = false; }
-            }
+            const uint32_t x = sw_lds_load(&ring[a & (SW_RING - 1)]);
+            const uint32_t y = sw_lds_load(&ring[b & (SW_RING - 1)]);
+            if (needA && x != SW_NOLEV) { la = x; needA = false; }
+            if (needB && y != SW_NOLEV) { lb = y; needB = false; }
             if (!needA && !needB) {
                 val = 1u + (la > lb ? la : lb);
                 __hip_atomic_store(&ring[p & (SW_RING - 1)], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
