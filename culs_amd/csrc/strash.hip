@@ -930,6 +930,11 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
     for (uint32_t i = tid; i < (uint32_t)SW_RING; i += SW_WG) ring[i] = SW_NOLEV;
     s_fan[0][tid] = tid < nPos ? make_uint2(fa[tid], fb[tid]) : make_uint2(NOFAN, NOFAN);
     uint32_t mx = 0;
+    // the level of this thread's position in the previous chunk, stored one
+    // chunk late: stores count on vmcnt like loads, so a store issued at the
+    // end of a chunk would hold up the next chunk's first wait
+    uint32_t pendP = 0, pendV = 0;
+    bool pend = false;
     __syncthreads();
     for (uint32_t c = 0, h = 0; c < nPos; c += SW_WG, h ^= 1u) {
         const uint32_t p = c + tid;
@@ -951,6 +956,7 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
         // (staged in LDS at the end of the chunk): their latency overlaps this
         // chunk's work, and nothing inside the wait loop below waits on them
         __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
+        if (pend) lev[pendP] = pendV;
         const uint32_t q = p + SW_WG;
         const uint32_t na = q < nPos ? fa[q] : NOFAN;
         const uint32_t nb = q < nPos ? fb[q] : NOFAN;
@@ -981,12 +987,15 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
             }
         }
         if (bad) atomicOr(&s_bad, 1u);
-        if (active) lev[p] = val;
+        pend = active;
+        pendP = p;
+        pendV = val;
         mx = val > mx ? val : mx;
         s_fan[h ^ 1u][tid] = make_uint2(na, nb);
         __syncthreads();
         if (s_bad) break;
     }
+    if (pend) lev[pendP] = pendV;
     atomicMax(&s_max, mx);
     __syncthreads();
     if (tid == 0) {
