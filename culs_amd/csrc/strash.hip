@@ -926,6 +926,18 @@ __global__ void __launch_bounds__(SW_WG) k_level_sweep(uint32_t nPos, const uint
     __shared__ uint2 s_fan[2][SW_WG];           // fanin positions of this chunk and the next
     __shared__ uint32_t s_max, s_bad;
     const uint32_t tid = threadIdx.x;
+    // every wait is on a lower position, so a lower wavefront never waits on a
+    // higher one: lower wavefronts issue first on their SIMD, so the spinning
+    // of the waiting ones does not slow the one they wait for (wavefronts
+    // w, w + 4, w + 8, w + 12 share a SIMD under round-robin placement)
+    // (a wavefront-uniform value, so the branches are scalar: s_setprio runs
+    // whatever the exec mask holds)
+    switch (__builtin_amdgcn_readfirstlane(tid >> 8)) {
+    case 0: __builtin_amdgcn_s_setprio(3); break;
+    case 1: __builtin_amdgcn_s_setprio(2); break;
+    case 2: __builtin_amdgcn_s_setprio(1); break;
+    default: break;
+    }
     if (tid == 0) { s_max = 0; s_bad = 0; }
     for (uint32_t i = tid; i < (uint32_t)SW_RING; i += SW_WG) ring[i] = SW_NOLEV;
     s_fan[0][tid] = tid < nPos ? make_uint2(fa[tid], fb[tid]) : make_uint2(NOFAN, NOFAN);
