@@ -243,6 +243,12 @@ constexpr int CUTL_BLOCK = 64;
 enum : int { C_SKIP = 0, C_FAIL = 1, C_OK = 2 };
 
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// x with lane l replaced by the wave-uniform v: the llvm.amdgcn.writelane
+// intrinsic (v_writelane_b32), which this compiler has no builtin name for
+extern "C" __device__ int gls_writelane_i32(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ uint32_t wl(uint32_t v, int l, uint32_t x) {
+    return (uint32_t)gls_writelane_i32((int)v, l, (int)x);
+}
 
 // Minimum of v over lanes 0..7, valid in lane 0: two quad_perm steps (xor 1,
 // xor 2) and row rotations by 4 and 12, so lane 0 also sees lanes 4..7 in
@@ -436,24 +442,24 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         const uint32_t n0 = (uint32_t)rl((int)m.leaves[0], l), n1 = (uint32_t)rl((int)m.leaves[1], l);
         const uint32_t n2 = (uint32_t)rl((int)m.leaves[2], l), n3 = (uint32_t)rl((int)m.leaves[3], l);
         const uint32_t nsig = (uint32_t)rl((int)msig, l);
-        const bool at = lane == id;
-        s.l0 = at ? n0 : s.l0; s.l1 = at ? n1 : s.l1; s.l2 = at ? n2 : s.l2; s.l3 = at ? n3 : s.l3;
-        s.n = at ? nn : s.n; s.sig = at ? nsig : s.sig;
+        // slot id takes the values (all wave-uniform): one v_writelane each
+        s.l0 = wl(n0, id, s.l0); s.l1 = wl(n1, id, s.l1); s.l2 = wl(n2, id, s.l2); s.l3 = wl(n3, id, s.l3);
+        s.n = (int)wl((uint32_t)nn, id, (uint32_t)s.n); s.sig = wl(nsig, id, s.sig);
         used |= 1u << id;
         if (filter_wave(s, lane, id, n0, n1, n2, n3, nn, nsig, used)) continue;
         const uint32_t ttw = (uint32_t)rl((int)tts, l);
-        s.tt = at ? ttw & 0xFFFFu : s.tt;
+        s.tt = wl(ttw & 0xFFFFu, id, s.tt);
         int fn = nn;
         if ((inf >> 2) & 1u) {
             fn = (int)((inf >> 6) & 7u);
             const uint32_t q0 = (uint32_t)rl((int)mm.leaves[0], l), q1 = (uint32_t)rl((int)mm.leaves[1], l);
             const uint32_t q2 = (uint32_t)rl((int)mm.leaves[2], l), q3 = (uint32_t)rl((int)mm.leaves[3], l);
             const uint32_t qsig = (uint32_t)rl((int)mmsig, l);
-            s.l0 = at ? q0 : s.l0; s.l1 = at ? q1 : s.l1; s.l2 = at ? q2 : s.l2; s.l3 = at ? q3 : s.l3;
-            s.n = at ? fn : s.n; s.tt = at ? ttw >> 16 : s.tt; s.sig = at ? qsig : s.sig;
+            s.l0 = wl(q0, id, s.l0); s.l1 = wl(q1, id, s.l1); s.l2 = wl(q2, id, s.l2); s.l3 = wl(q3, id, s.l3);
+            s.n = (int)wl((uint32_t)fn, id, (uint32_t)s.n); s.tt = wl(ttw >> 16, id, s.tt); s.sig = wl(qsig, id, s.sig);
             filter_wave(s, lane, id, q0, q1, q2, q3, fn, qsig, used);
         }
-        s.value = at ? (inf >> 9) & 0x7FFu : s.value;
+        s.value = wl((inf >> 9) & 0x7FFu, id, s.value);
         if (fn < 2) break;
     }
     // ---- phase 3: lanes 0..7 store the slots
