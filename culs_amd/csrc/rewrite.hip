@@ -22,6 +22,7 @@
 #include "gls_internal.h"
 #include "dev_common.h"
 #include "rwlib_forest.h"
+#include "tt_place.h"
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 #include <cstdlib>
@@ -149,49 +150,50 @@ __device__ __forceinline__ int merge_ordered(const DCut &a, const DCut &b, DCut 
     return (i >= a.nLeaves) & (k >= b.nLeaves);
 }
 
-// swaps truth-table variables v and v + 1 (v in 0..2), without branches
-__device__ __forceinline__ uint32_t swap_adj(uint32_t t, int v) {
-    const uint32_t up = v == 0 ? 0x22222222u : v == 1 ? 0x0C0C0C0Cu : 0x00F000F0u;
-    const uint32_t sh = 1u << v;
-    const uint32_t dn = up << sh;
-    return (t & ~(up | dn)) | ((t & up) << sh) | ((t & dn) >> sh);
+// Placement of truth-table variables (tt_place.h): one entry per phase,
+// held in lanes 0..15 of two registers and read with ds_bpermute, so a lane
+// finds its phase's swaps without a memory access and without a loop.
+__constant__ uint64_t kTtPlace[16] = {
+    tt_place_entry(0),  tt_place_entry(1),  tt_place_entry(2),  tt_place_entry(3),
+    tt_place_entry(4),  tt_place_entry(5),  tt_place_entry(6),  tt_place_entry(7),
+    tt_place_entry(8),  tt_place_entry(9),  tt_place_entry(10), tt_place_entry(11),
+    tt_place_entry(12), tt_place_entry(13), tt_place_entry(14), tt_place_entry(15)};
+
+struct PlaceTab {
+    uint32_t lo, hi;    // lane i (i < 16): kTtPlace[i]
+};
+
+__device__ __forceinline__ PlaceTab place_tab(int lane) {
+    const uint64_t e = kTtPlace[lane & 15];
+    return PlaceTab{(uint32_t)e, (uint32_t)(e >> 32)};
 }
 
-__device__ uint32_t tt_stretch(uint32_t t, int nVar, int phase) {
-    int var = nVar - 1;
-    for (int i = 3; i >= 0; i--)
-        if ((phase >> i) & 1) {
-            for (int k = var; k < i; k++) t = swap_adj(t, k);
-            var--;
-        }
-    return t;
+// the entry of `phase` (0..15); every lane of the wavefront must be active
+__device__ __forceinline__ uint64_t place_of(const PlaceTab &tab, int phase) {
+    const int a = phase << 2;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)tab.lo);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)tab.hi);
+    return (uint64_t)hi << 32 | lo;
 }
 
-__device__ uint32_t tt_shrink(uint32_t t, int phase) {
-    int var = 0;
-    for (int i = 0; i < 4; i++)
-        if (phase & (1 << i)) {
-            for (int k = i - 1; k >= var; k--) t = swap_adj(t, k);
-            var++;
-        }
-    return t;
-}
-
-__device__ __forceinline__ int minimize_support(DCut &c) {
+// Support reduction (rewrite.cu:223-266): the leaves the table does not
+// depend on are dropped, the others keep their order, and the table's
+// variables move down with them.  Without branches: when every leaf stays,
+// the shrink has no swaps and the compaction keeps the leaves where they are
+// (positions past the count already repeat the first leaf, as after the
+// merge).  Returns 1 if the cut shrank.
+__device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab) {
     const uint32_t m0[4] = {0x5555, 0x3333, 0x0F0F, 0x00FF};
     const uint32_t m1[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
-    int phase = 0, nL = c.nLeaves;
-    uint32_t t = c.tt & 0xFFFF;
-    for (int i = 0; i < 4; i++) {
-        if (i >= c.nLeaves) break;
-        if ((t & m0[i]) == ((t & m1[i]) >> (1 << i))) nL--;
-        else phase |= 1 << i;
-    }
-    if (nL == c.nLeaves) return 0;
-    t = tt_shrink(t, phase);
-    c.tt = t & 0xFFFF;
+    const uint32_t t = c.tt & 0xFFFF;
+    int phase = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        phase |= (int)((i < c.nLeaves) & ((t & m0[i]) != ((t & m1[i]) >> (1 << i)))) << i;
+    const int nL = __popc((uint32_t)phase);
+    c.tt = tt_place_shrink(t, place_of(tab, phase)) & 0xFFFF;
     // compaction: kept leaf i moves to slot popc(phase below i); the slots past
-    // the new count repeat the first leaf, as after the merge
+    // the new count repeat the first leaf
     uint32_t o[4];
 #pragma unroll
     for (int p = 0; p < 4; p++) o[p] = c.leaves[p];
@@ -204,8 +206,9 @@ __device__ __forceinline__ int minimize_support(DCut &c) {
     }
 #pragma unroll
     for (int p = 0; p < 4; p++) c.leaves[p] = p < nL ? o[p] : o[0];
+    const int shrunk = nL != c.nLeaves;
     c.nLeaves = nL;
-    return 1;
+    return shrunk;
 }
 
 __device__ void store_cuts(uint32_t *out, const DCut *cs) {
@@ -361,6 +364,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
                                          const int32_t *__restrict__ nRef, unsigned long long *tcuts, uint32_t *err,
                                          const uint8_t *__restrict__ sigBits, uint32_t *out2 = nullptr) {
     const uint32_t in0 = LID(l0), in1 = LID(l1), c0 = LC(l0), c1 = LC(l1);
+    const PlaceTab tab = place_tab(lane);
     DCut a, b;
     if (MODE == REC_TAGGED) {
         const unsigned long long *ra = tcuts + ((size_t)in0 * NCUT + (lane >> 3)) * 3;
@@ -394,11 +398,11 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         int phA, phB;
         const int ok = merge_ordered(a, b, m, phA, phB);
         uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
-        t0 = tt_stretch(t0, a.nLeaves, phA);
-        t1 = tt_stretch(t1, b.nLeaves, phB);
+        t0 = tt_place_stretch(t0, place_of(tab, phA));
+        t1 = tt_place_stretch(t1, place_of(tab, phB));
         m.tt = (t0 & t1) & 0xFFFF;
         mm = m;
-        shrunk = minimize_support(mm);
+        shrunk = minimize_support(mm, tab);
         mm.value = cut_value(mm, nRef);
         status = !cand ? C_SKIP : !ok ? C_FAIL : C_OK;
     }

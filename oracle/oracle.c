@@ -843,6 +843,15 @@ static uint32_t tt_shrink(uint32_t t, int phase) {                /* rewrite.cu:
     return t;
 }
 
+/* The two variable placements above over arrays, for the tests of the
+ * library's delta-swap form (tt_place.h): out[i] = tt_stretch(t[i],
+ * popcount(phase[i]), phase[i]) when shrink == 0, tt_shrink(t[i], phase[i])
+ * otherwise. */
+void orc_tt_place(int n, const uint32_t *t, const uint8_t *phase, int shrink, uint32_t *out) {
+    for (int i = 0; i < n; i++)
+        out[i] = shrink ? tt_shrink(t[i], phase[i]) : tt_stretch(t[i], __builtin_popcount(phase[i]), phase[i]);
+}
+
 static int minimize_support(Cut *c) {                             /* rewrite.cu:243-266 */
     static const uint32_t masks[4][2] = {{0x5555, 0xAAAA}, {0x3333, 0xCCCC}, {0x0F0F, 0xF0F0}, {0x00FF, 0xFF00}};
     int phase = 0, nL = c->nLeaves;

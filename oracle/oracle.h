@@ -46,6 +46,7 @@ int orc_balance_ex(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, 
 /* thrust::shuffle(thrust::seq, first, first + len, default_random_engine(seed))
  * as a permutation: output position k holds input element perm[k]. */
 void orc_thrust_shuffle_perm(uint32_t len, uint32_t seed, uint32_t *perm);
+void orc_tt_place(int n, const uint32_t *t, const uint8_t *phase, int shrink, uint32_t *out);
 
 /* Level-major renumbering (ties by id) of an AIG whose ids are topological. */
 int orc_reorder(int nObjs, int nPIs, const uint32_t *f0, const uint32_t *f1, int nPOs,

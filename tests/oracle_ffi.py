@@ -120,6 +120,16 @@ def thrust_shuffle_perm(length, seed):
     return perm[:length]
 
 
+def tt_place(t, phase, shrink):
+    """The oracle's truth-table variable placement (tt_stretch / tt_shrink of
+    rewrite.cu:213-241) over arrays."""
+    t = u32(t)
+    ph = np.ascontiguousarray(phase, dtype=np.uint8)
+    out = np.zeros(len(t), np.uint32)
+    lib().orc_tt_place(len(t), P(t), P(ph), int(shrink), P(out))
+    return out
+
+
 def reorder(nPIs, f0, f1, pos):
     f0, f1, pos = u32(f0), u32(f1), u32(pos)
     n = len(f0)
