@@ -178,11 +178,14 @@ __device__ __forceinline__ uint64_t place_of(const PlaceTab &tab, int phase) {
 
 // Support reduction (rewrite.cu:223-266): the leaves the table does not
 // depend on are dropped, the others keep their order, and the table's
-// variables move down with them.  Without branches: when every leaf stays,
-// the shrink has no swaps and the compaction keeps the leaves where they are
-// (positions past the count already repeat the first leaf, as after the
-// merge).  Returns 1 if the cut shrank.
-__device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab) {
+// variables move down with them.  Without branches per lane: when every leaf
+// stays, the shrink has no swaps and the compaction keeps the leaves where
+// they are (positions past the count already repeat the first leaf, as after
+// the merge).  The shrink and the compaction run only if some lane whose
+// result is used (`used_here`) shrinks; otherwise every such lane's cut is
+// already its result, and the others' results are not read.  Returns 1 if
+// the cut shrank (0 in lanes left as they were).
+__device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab, bool used_here) {
     const uint32_t m0[4] = {0x5555, 0x3333, 0x0F0F, 0x00FF};
     const uint32_t m1[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
     const uint32_t t = c.tt & 0xFFFF;
@@ -191,6 +194,7 @@ __device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab) {
     for (int i = 0; i < 4; i++)
         phase |= (int)((i < c.nLeaves) & ((t & m0[i]) != ((t & m1[i]) >> (1 << i)))) << i;
     const int nL = __popc((uint32_t)phase);
+    if (__ballot(used_here && nL != c.nLeaves) == 0) return 0;     // wave-uniform
     c.tt = tt_place_shrink(t, place_of(tab, phase)) & 0xFFFF;
     // compaction: kept leaf i moves to slot popc(phase below i); the slots past
     // the new count repeat the first leaf
@@ -402,7 +406,7 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         t1 = tt_place_stretch(t1, place_of(tab, phB));
         m.tt = (t0 & t1) & 0xFFFF;
         mm = m;
-        shrunk = minimize_support(mm, tab);
+        shrunk = minimize_support(mm, tab, cand && ok);
         mm.value = cut_value(mm, nRef);
         status = !cand ? C_SKIP : !ok ? C_FAIL : C_OK;
     }
