@@ -85,11 +85,32 @@ __device__ __forceinline__ uint32_t cut_sign_of(const DCut &c, const uint8_t *__
     return s;
 }
 
-__device__ __forceinline__ uint32_t cut_value(const DCut &c, const int32_t *nRef) {
-    // all four loads in flight at once; absent leaves read node 0's entry
-    int r[4];
+// the reference counts of a cut's leaves, all four loads in flight at once;
+// absent leaves read node 0's entry
+__device__ __forceinline__ void leaf_refs(const DCut &c, const int32_t *nRef, int (&r)[4]) {
 #pragma unroll
     for (int i = 0; i < 4; i++) r[i] = nRef[i < c.nLeaves ? c.leaves[i] : 0u];
+}
+
+// CutFindValue (rewrite.cu:49-61) of a cut of nLeaves leaves whose reference
+// counts are r[0..nLeaves-1]
+__device__ __forceinline__ uint32_t cut_value_of(int nLeaves, const int (&r)[4]) {
+    int value = 0, nOnes = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (i < nLeaves) {
+            value += r[i];
+            nOnes += (r[i] == 1);
+        }
+    if (nLeaves < 2) return 1001;
+    if (value > 1000) value = 1000;
+    if (nOnes > 3) value = 5 - nOnes;
+    return (uint32_t)value;
+}
+
+__device__ __forceinline__ uint32_t cut_value(const DCut &c, const int32_t *nRef) {
+    int r[4];
+    leaf_refs(c, nRef, r);
     int value = 0, nOnes = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -183,9 +204,10 @@ __device__ __forceinline__ uint64_t place_of(const PlaceTab &tab, int phase) {
 // they are (positions past the count already repeat the first leaf, as after
 // the merge).  The shrink and the compaction run only if some lane whose
 // result is used (`used_here`) shrinks; otherwise every such lane's cut is
-// already its result, and the others' results are not read.  Returns 1 if
-// the cut shrank (0 in lanes left as they were).
-__device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab, bool used_here) {
+// already its result, and the others' results are not read.  r holds the
+// leaves' reference counts and moves with them.  Returns 1 if the cut shrank
+// (0 in lanes left as they were).
+__device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab, bool used_here, int (&r)[4]) {
     const uint32_t m0[4] = {0x5555, 0x3333, 0x0F0F, 0x00FF};
     const uint32_t m1[4] = {0xAAAA, 0xCCCC, 0xF0F0, 0xFF00};
     const uint32_t t = c.tt & 0xFFFF;
@@ -199,17 +221,18 @@ __device__ __forceinline__ int minimize_support(DCut &c, const PlaceTab &tab, bo
     // compaction: kept leaf i moves to slot popc(phase below i); the slots past
     // the new count repeat the first leaf
     uint32_t o[4];
+    int q[4];
 #pragma unroll
-    for (int p = 0; p < 4; p++) o[p] = c.leaves[p];
+    for (int p = 0; p < 4; p++) { o[p] = c.leaves[p]; q[p] = r[p]; }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int pos = __popc((uint32_t)phase & ((1u << i) - 1u));
 #pragma unroll
         for (int p = 0; p < 4; p++)
-            if (p == pos && ((phase >> i) & 1)) o[p] = c.leaves[i];
+            if (p == pos && ((phase >> i) & 1)) { o[p] = c.leaves[i]; q[p] = r[i]; }
     }
 #pragma unroll
-    for (int p = 0; p < 4; p++) c.leaves[p] = p < nL ? o[p] : o[0];
+    for (int p = 0; p < 4; p++) { c.leaves[p] = p < nL ? o[p] : o[0]; r[p] = q[p]; }
     const int shrunk = nL != c.nLeaves;
     c.nLeaves = nL;
     return shrunk;
@@ -401,13 +424,17 @@ __device__ __forceinline__ bool cut_node(const int idx, const int lane, const ui
         const bool cand = a.used && b.used && __popc(cut_sign_of(a, sigBits) | cut_sign_of(b, sigBits)) <= 4;
         int phA, phB;
         const int ok = merge_ordered(a, b, m, phA, phB);
+        // the merged leaves' reference counts, in flight during the truth
+        // tables; a reduced cut's leaves are some of them
+        int r[4];
+        leaf_refs(m, nRef, r);
         uint32_t t0 = c0 ? ~a.tt : a.tt, t1 = c1 ? ~b.tt : b.tt;
         t0 = tt_place_stretch(t0, place_of(tab, phA));
         t1 = tt_place_stretch(t1, place_of(tab, phB));
         m.tt = (t0 & t1) & 0xFFFF;
         mm = m;
-        shrunk = minimize_support(mm, tab, cand && ok);
-        mm.value = cut_value(mm, nRef);
+        shrunk = minimize_support(mm, tab, cand && ok, r);
+        mm.value = cut_value_of(mm.nLeaves, r);
         status = !cand ? C_SKIP : !ok ? C_FAIL : C_OK;
     }
     // per-candidate words read by the replay: status, shrink flag, leaf counts
